@@ -1,0 +1,501 @@
+// C ABI of libtpm (declared in include/tpm.h): context lifecycle, the full force evaluation
+// that replaces Simulation::accelerations (src/integrator.rs:140-161), and the component
+// entry points that mirror the reference's public functions.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "tpm_internal.h"
+
+namespace tpm {
+size_t sort_temp_bytes(size_t max_n);
+
+static std::string g_create_error;
+
+__global__ void k_inverse_perm(const uint32_t *__restrict__ idx, size_t n, uint32_t *__restrict__ inv) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[idx[i]] = (uint32_t)i;
+}
+
+__global__ void k_map_targets(const uint32_t *__restrict__ inv, const uint32_t *__restrict__ caller, size_t m,
+                              size_t n, uint32_t *__restrict__ out, int *bad) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const uint32_t c = caller[t];
+    if (c >= n) {
+        *bad = 1;
+        out[t] = 0;
+        return;
+    }
+    out[t] = inv[c];
+}
+
+__global__ void k_add3(double *__restrict__ a, const double *__restrict__ b, size_t count) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) a[i] += b[i];
+}
+
+static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+static void validate(const tpm_params &p) {
+    TPM_REQUIRE(p.mesh_n >= 2 && p.mesh_n <= 1024, "mesh_n must be in [2, 1024]");
+    TPM_REQUIRE(std::isfinite(p.box_size) && p.box_size > 0, "box_size must be positive");
+    TPM_REQUIRE(std::isfinite(p.omega_m), "omega_m must be finite");
+    TPM_REQUIRE(std::isfinite(p.r_split) && p.r_split >= 0, "r_split must be >= 0");
+    if (p.flags & TPM_FLAG_TREEPM) {
+        TPM_REQUIRE(p.r_split > 0, "TreePM needs r_split > 0");
+        TPM_REQUIRE(std::isfinite(p.r_cut) && p.r_cut > 0, "TreePM needs r_cut > 0");
+        TPM_REQUIRE(std::isfinite(p.softening) && p.softening >= 0, "softening must be >= 0");
+        TPM_REQUIRE(std::isfinite(p.theta) && p.theta >= 0, "theta must be >= 0");
+    }
+}
+
+static void free_all(Ctx &c) {
+    void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.sort_tmp, c.spos,
+                    c.counter, c.flag, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
+                    c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.scan_tmp,
+                    c.scan_tmp_storage, c.inv, c.judge_buf};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
+static void create(Ctx &c, const tpm_params &p, size_t max_n) {
+    validate(p);
+    TPM_REQUIRE(max_n < (size_t)0xFFFFFFF0u, "max_particles must fit 32-bit indices");
+    c.p = p;
+    c.max_n = max_n;
+    c.n = p.mesh_n;
+    c.hp = c.n / 2 + 1;
+    c.m = (size_t)c.n * c.n * c.n;
+    c.hs = (size_t)c.n * c.n * c.hp;
+    const size_t mn = max_n ? max_n : 1;
+    TPM_HIP(hipStreamCreate(&c.stream));
+    TPM_HIP(hipMalloc(&c.tw32, c.n * sizeof(float2)));
+    TPM_HIP(hipMalloc(&c.tw64, c.n * sizeof(double2)));
+    TPM_HIP(hipMalloc(&c.kvec, c.n * sizeof(double)));
+    TPM_HIP(hipMalloc(&c.pos, 3 * mn * sizeof(double)));
+    TPM_HIP(hipMalloc(&c.acc, 3 * mn * sizeof(float)));
+    TPM_HIP(hipMalloc(&c.keys, mn * sizeof(uint64_t)));
+    TPM_HIP(hipMalloc(&c.keys_alt, mn * sizeof(uint64_t)));
+    TPM_HIP(hipMalloc(&c.idx, mn * sizeof(uint32_t)));
+    TPM_HIP(hipMalloc(&c.idx_alt, mn * sizeof(uint32_t)));
+    c.sort_tmp_bytes = sort_temp_bytes(mn);
+    TPM_HIP(hipMalloc(&c.sort_tmp, c.sort_tmp_bytes ? c.sort_tmp_bytes : 16));
+    TPM_HIP(hipMalloc(&c.spos, 3 * mn * sizeof(double)));
+    TPM_HIP(hipMalloc(&c.counter, 8 * sizeof(unsigned long long)));
+    TPM_HIP(hipMalloc(&c.flag, sizeof(int) * 4));
+    TPM_HIP(hipMalloc(&c.grid, c.m * sizeof(unsigned long long)));
+    TPM_HIP(hipMalloc(&c.specA, c.hs * sizeof(float2)));
+    TPM_HIP(hipMalloc(&c.specB, c.hs * sizeof(float2)));
+    TPM_HIP(hipMalloc(&c.specC, c.hs * sizeof(float2)));
+    TPM_HIP(hipMalloc(&c.force, 3 * c.m * sizeof(float)));
+    TPM_HIP(hipMalloc(&c.delta, c.m * sizeof(float)));
+    make_plan(c);
+}
+
+// Stage timer: records an event on the context stream when timing is on.
+struct Timer {
+    Ctx &c;
+    std::vector<hipEvent_t> ev;
+    explicit Timer(Ctx &cc) : c(cc) {}
+    void mark() {
+        if (!c.timing) return;
+        hipEvent_t e;
+        TPM_HIP(hipEventCreate(&e));
+        TPM_HIP(hipEventRecord(e, c.stream));
+        ev.push_back(e);
+    }
+    double ms(size_t a, size_t b) {
+        if (!c.timing || b >= ev.size()) return 0.0;
+        float t = 0;
+        TPM_HIP(hipEventElapsedTime(&t, ev[a], ev[b]));
+        return t;
+    }
+    ~Timer() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+// integrator.rs:150-152: K = (3/2) Omega_m L^3 / (4 pi a N); in FP32 for the FP32 interface.
+static double coupling(const Ctx &c, double a, size_t n, bool f32) {
+    if (f32) {
+        const float l = (float)c.p.box_size;
+        const float l3 = l * l * l;
+        return (double)(1.5f * (float)c.p.omega_m * l3 / (4.0f * 3.14159265358979323846f * (float)a * (float)n));
+    }
+    const double l = c.p.box_size;
+    return 1.5 * c.p.omega_m * l * l * l / (4.0 * M_PI * a * (double)n);
+}
+
+// The force evaluation on device-resident positions (caller order, FP64).
+static void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float *d_acc, float *d_delta) {
+    TPM_REQUIRE(n <= c.max_n, "more particles than the context capacity");
+    TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
+    const bool treepm = (c.p.flags & TPM_FLAG_TREEPM) != 0;
+    Timer tm(c);
+    TPM_HIP(hipMemsetAsync(c.counter, 0, 8 * sizeof(unsigned long long), c.stream));
+    tm.mark();
+    sort_particles(c, d_pos, n);
+    tm.mark();
+    deposit_fixed(c, n, 1.0, f32w, c.spos);
+    tm.mark();
+    if (n > 0) {
+        pm_from_grid(c, n, a, treepm ? c.p.r_split : 0.0, d_delta);
+    } else {
+        if (d_delta) TPM_HIP(hipMemsetAsync(d_delta, 0, c.m * sizeof(float), c.stream));
+    }
+    tm.mark();
+    interp_force(c, n, f32w, d_acc);
+    tm.mark();
+    if (treepm && n > 0) {
+        build_tree(c, n);
+        tm.mark();
+        walk_short_range(c, n, coupling(c, a, n, f32w), d_acc);
+        tm.mark();
+    } else {
+        tm.mark();
+        tm.mark();
+    }
+    if (c.p.flags & TPM_FLAG_CHECK_FINITE) check_finite(c, d_acc, 3 * n);
+    if (c.timing) {
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        c.stats.ms_sort = tm.ms(0, 1);
+        c.stats.ms_deposit = tm.ms(1, 2);
+        c.stats.ms_fft = tm.ms(2, 3);
+        c.stats.ms_interp = tm.ms(3, 4);
+        c.stats.ms_tree = tm.ms(4, 5);
+        c.stats.ms_walk = tm.ms(5, 6);
+        c.stats.ms_total = tm.ms(0, 6);
+        unsigned long long cnt[2] = {0, 0};
+        TPM_HIP(hipMemcpy(cnt, c.counter, sizeof(cnt), hipMemcpyDeviceToHost));
+        c.stats.interactions = cnt[0];
+        c.stats.nodes = c.tree.count;
+        if (cnt[1]) throw Error{TPM_E_DEVICE, "walk stack overflow"};
+    }
+}
+
+static void finish_checks(Ctx &c) {
+    unsigned long long ovf = 0;
+    TPM_HIP(hipMemcpyAsync(&ovf, c.counter + 1, sizeof(ovf), hipMemcpyDeviceToHost, c.stream));
+    TPM_HIP(hipStreamSynchronize(c.stream));
+    if (ovf) throw Error{TPM_E_DEVICE, "walk stack overflow"};
+    if (c.p.flags & TPM_FLAG_CHECK_FINITE) {
+        int bad = 0;
+        TPM_HIP(hipMemcpyAsync(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        if (bad) throw Error{TPM_E_NONFINITE, "non-finite acceleration"};
+    }
+}
+
+// Copies caller positions into c.pos as FP64 (f32 inputs are widened exactly).
+static void upload_positions(Ctx &c, const void *pos, size_t n, bool f32) {
+    TPM_REQUIRE(n <= c.max_n, "more particles than the context capacity");
+    if (n == 0) return;
+    TPM_REQUIRE(pos != nullptr, "positions pointer is null");
+    if (f32) {
+        TPM_HIP(hipMemcpyAsync(c.acc, pos, 3 * n * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        convert_f32_to_f64(c, c.acc, c.pos, 3 * n);
+    } else {
+        TPM_HIP(hipMemcpyAsync(c.pos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    }
+}
+
+static int run_host_accel(Ctx &c, const void *pos, size_t n, double a, float *acc, float *delta, bool f32) {
+    TPM_REQUIRE(n == 0 || acc != nullptr, "acc pointer is null");
+    upload_positions(c, pos, n, f32);
+    evaluate(c, c.pos, n, a, f32, c.acc, delta ? c.delta : nullptr);
+    finish_checks(c);
+    if (n) TPM_HIP(hipMemcpyAsync(acc, c.acc, 3 * n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    if (delta) TPM_HIP(hipMemcpyAsync(delta, c.delta, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    TPM_HIP(hipStreamSynchronize(c.stream));
+    return TPM_OK;
+}
+
+}  // namespace tpm
+
+using tpm::Ctx;
+
+struct tpm_ctx {
+    Ctx c;
+};
+
+template <typename F>
+static int guarded(tpm_ctx *ctx, F &&f) {
+    if (!ctx) return TPM_E_INVALID;
+    try {
+        return f(ctx->c);
+    } catch (const tpm::Error &e) {
+        ctx->c.err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        ctx->c.err = "host allocation failed";
+        return TPM_E_DEVICE;
+    }
+}
+
+extern "C" {
+
+const char *tpm_version(void) { return "tpm 0.1 (gfx950)"; }
+
+int tpm_create(const tpm_params *params, size_t max_particles, tpm_ctx **out) {
+    if (!params || !out) {
+        tpm::g_create_error = "null argument";
+        return TPM_E_INVALID;
+    }
+    *out = nullptr;
+    tpm_ctx *ctx = new (std::nothrow) tpm_ctx();
+    if (!ctx) {
+        tpm::g_create_error = "host allocation failed";
+        return TPM_E_DEVICE;
+    }
+    try {
+        tpm::create(ctx->c, *params, max_particles);
+    } catch (const tpm::Error &e) {
+        tpm::g_create_error = e.msg;
+        tpm::free_all(ctx->c);
+        delete ctx;
+        return e.code;
+    }
+    *out = ctx;
+    return TPM_OK;
+}
+
+void tpm_destroy(tpm_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->c.stream);
+    tpm::free_all(ctx->c);
+    delete ctx;
+}
+
+const char *tpm_last_error(const tpm_ctx *ctx) {
+    if (!ctx) return tpm::g_create_error.c_str();
+    return ctx->c.err.c_str();
+}
+
+int tpm_set_params(tpm_ctx *ctx, const tpm_params *params) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(params != nullptr, "null params");
+        tpm::validate(*params);
+        TPM_REQUIRE(params->mesh_n == c.p.mesh_n, "mesh_n cannot change on a context");
+        const bool box_changed = params->box_size != c.p.box_size;
+        c.p = *params;
+        if (box_changed) tpm::make_plan(c);
+        return TPM_OK;
+    });
+}
+
+int tpm_set_timing(tpm_ctx *ctx, int enabled) {
+    return guarded(ctx, [&](Ctx &c) {
+        c.timing = enabled != 0;
+        return TPM_OK;
+    });
+}
+
+int tpm_get_stats(const tpm_ctx *ctx, tpm_stats *out) {
+    if (!ctx || !out) return TPM_E_INVALID;
+    *out = ctx->c.stats;
+    return TPM_OK;
+}
+
+int tpm_accelerations_f32(tpm_ctx *ctx, const float *pos, size_t n, double a, float *acc, float *delta) {
+    return guarded(ctx, [&](Ctx &c) { return tpm::run_host_accel(c, pos, n, a, acc, delta, true); });
+}
+
+int tpm_accelerations_f64(tpm_ctx *ctx, const double *pos, size_t n, double a, float *acc, float *delta) {
+    return guarded(ctx, [&](Ctx &c) { return tpm::run_host_accel(c, pos, n, a, acc, delta, false); });
+}
+
+int tpm_accelerations_dev_f64(tpm_ctx *ctx, const double *d_pos, size_t n, double a, float *d_acc, float *d_delta,
+                              void *stream) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(n == 0 || (d_pos && d_acc), "null device pointer");
+        hipStream_t saved = c.stream;
+        if (stream) c.stream = (hipStream_t)stream;
+        try {
+            tpm::evaluate(c, d_pos, n, a, false, d_acc, d_delta);
+        } catch (...) {
+            c.stream = saved;
+            throw;
+        }
+        c.stream = saved;
+        return TPM_OK;
+    });
+}
+
+int tpm_deposit_f32(tpm_ctx *ctx, const float *pos, size_t n, float mass, float *cells) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(cells != nullptr, "cells pointer is null");
+        tpm::upload_positions(c, pos, n, true);
+        tpm::deposit_fixed(c, n, (double)mass, true, c.pos);
+        tpm::grid_to_float(c, n ? n : 1, nullptr, c.delta);
+        TPM_HIP(hipMemcpyAsync(cells, c.delta, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        return TPM_OK;
+    });
+}
+
+int tpm_deposit_f64(tpm_ctx *ctx, const double *pos, size_t n, double mass, float *cells) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(cells != nullptr, "cells pointer is null");
+        tpm::upload_positions(c, pos, n, false);
+        tpm::deposit_fixed(c, n, mass, false, c.pos);
+        tpm::grid_to_float(c, n ? n : 1, nullptr, c.delta);
+        TPM_HIP(hipMemcpyAsync(cells, c.delta, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        return TPM_OK;
+    });
+}
+
+int tpm_density_contrast_f32(tpm_ctx *ctx, const float *pos, size_t n, float *delta) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(delta != nullptr, "delta pointer is null");
+        TPM_REQUIRE(n > 0, "density contrast needs at least one particle");
+        tpm::upload_positions(c, pos, n, true);
+        tpm::deposit_fixed(c, n, 1.0, true, c.pos);
+        tpm::grid_to_float(c, n, c.delta, nullptr);
+        TPM_HIP(hipMemcpyAsync(delta, c.delta, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        return TPM_OK;
+    });
+}
+
+int tpm_pm_accel(tpm_ctx *ctx, const float *delta, double a, double r_split, float *gx, float *gy, float *gz) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(delta && gx && gy && gz, "null grid pointer");
+        TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
+        TPM_REQUIRE(r_split >= 0 && std::isfinite(r_split), "r_split must be >= 0");
+        TPM_HIP(hipMemcpyAsync(c.delta, delta, c.m * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        tpm::pm_from_delta(c, c.delta, a, r_split);
+        TPM_HIP(hipMemcpyAsync(gx, c.force, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipMemcpyAsync(gy, c.force + c.m, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipMemcpyAsync(gz, c.force + 2 * c.m, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        return TPM_OK;
+    });
+}
+
+int tpm_interpolate_f32(tpm_ctx *ctx, const float *cells, const float *pos, size_t n, float *out) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(cells && (n == 0 || out), "null pointer");
+        TPM_HIP(hipMemcpyAsync(c.delta, cells, c.m * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        tpm::upload_positions(c, pos, n, true);
+        tpm::interp_one(c, c.delta, c.pos, n, true, c.acc);
+        if (n) TPM_HIP(hipMemcpyAsync(out, c.acc, n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        return TPM_OK;
+    });
+}
+
+static int short_range(tpm_ctx *ctx, const void *pos, size_t n, double k, float *acc, bool f32) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(n == 0 || acc != nullptr, "acc pointer is null");
+        TPM_REQUIRE(c.p.r_split > 0 && c.p.r_cut > 0, "short-range force needs r_split > 0 and r_cut > 0");
+        tpm::upload_positions(c, pos, n, f32);
+        if (n == 0) return TPM_OK;
+        TPM_HIP(hipMemsetAsync(c.counter, 0, 8 * sizeof(unsigned long long), c.stream));
+        tpm::sort_particles(c, c.pos, n);
+        TPM_HIP(hipMemsetAsync(c.acc, 0, 3 * n * sizeof(float), c.stream));
+        tpm::build_tree(c, n);
+        tpm::walk_short_range(c, n, k, c.acc);
+        TPM_HIP(hipMemcpyAsync(acc, c.acc, 3 * n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        unsigned long long cnt = 0;
+        TPM_HIP(hipMemcpyAsync(&cnt, c.counter, sizeof(cnt), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        c.stats.interactions = cnt;
+        c.stats.nodes = c.tree.count;
+        return TPM_OK;
+    });
+}
+
+int tpm_short_range_f32(tpm_ctx *ctx, const float *pos, size_t n, double k, float *acc) {
+    return short_range(ctx, pos, n, k, acc, true);
+}
+
+int tpm_short_range_f64(tpm_ctx *ctx, const double *pos, size_t n, double k, float *acc) {
+    return short_range(ctx, pos, n, k, acc, false);
+}
+
+int tpm_judge_f64(tpm_ctx *ctx, const double *pos, size_t n, const uint32_t *idx, size_t m, double a, double theta,
+                  double *acc) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(n > 0 && pos && idx && acc, "judge needs particles, targets and an output");
+        TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
+        TPM_REQUIRE(theta >= 0 && std::isfinite(theta), "theta must be >= 0");
+        tpm::upload_positions(c, pos, n, false);
+        tpm::sort_particles(c, c.pos, n);
+        tpm::deposit_fixed(c, n, 1.0, false, c.spos);
+        const bool treepm = (c.p.flags & TPM_FLAG_TREEPM) != 0;
+        tpm::pm_from_grid_f64(c, n, a, treepm ? c.p.r_split : 0.0);
+        if (!c.inv) TPM_HIP(hipMalloc(&c.inv, (c.max_n ? c.max_n : 1) * sizeof(uint32_t)));
+        if (c.judge_cap < m) {
+            if (c.judge_buf) (void)hipFree(c.judge_buf);
+            TPM_HIP(hipMalloc(&c.judge_buf, m * (sizeof(uint32_t) * 2 + 6 * sizeof(double)) + 256));
+            c.judge_cap = m;
+        }
+        uint32_t *d_caller = (uint32_t *)c.judge_buf;
+        uint32_t *d_sorted = d_caller + m;
+        double *d_pm = (double *)((char *)c.judge_buf + ((m * 2 * sizeof(uint32_t) + 15) / 16) * 16);
+        double *d_sr = d_pm + 3 * m;
+        TPM_HIP(hipMemcpyAsync(d_caller, idx, m * sizeof(uint32_t), hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(tpm::k_inverse_perm, dim3(tpm::blocks(n, 256)), dim3(256), 0, c.stream, c.idx, n, c.inv);
+        TPM_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+        hipLaunchKernelGGL(tpm::k_map_targets, dim3(tpm::blocks(m, 256)), dim3(256), 0, c.stream, c.inv, d_caller, m, n,
+                           d_sorted, c.flag);
+        TPM_HIP(hipGetLastError());
+        tpm::interp_judge_f64(c, d_sorted, m, d_pm);
+        if (treepm) {
+            tpm::build_tree(c, n);
+            const double k = 1.5 * c.p.omega_m * c.p.box_size * c.p.box_size * c.p.box_size / (4.0 * M_PI * a * (double)n);
+            tpm::walk_judge_f64(c, n, d_sorted, m, k, theta, d_sr);
+            hipLaunchKernelGGL(tpm::k_add3, dim3(tpm::blocks(3 * m, 256)), dim3(256), 0, c.stream, d_pm, d_sr, 3 * m);
+            TPM_HIP(hipGetLastError());
+        }
+        int bad = 0;
+        TPM_HIP(hipMemcpyAsync(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipMemcpyAsync(acc, d_pm, 3 * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        TPM_REQUIRE(!bad, "judge target index out of range");
+        return TPM_OK;
+    });
+}
+
+int tpm_zeldovich_dev(tpm_ctx *ctx, uint64_t seed, double k_c, double disp_rms, double mom_factor, double *d_pos,
+                      double *d_mom, void *stream) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(d_pos != nullptr, "null positions pointer");
+        hipStream_t saved = c.stream;
+        if (stream) c.stream = (hipStream_t)stream;
+        try {
+            tpm::zeldovich(c, seed, k_c, disp_rms, mom_factor, d_pos, d_mom);
+        } catch (...) {
+            c.stream = saved;
+            throw;
+        }
+        c.stream = saved;
+        return TPM_OK;
+    });
+}
+
+int tpm_zeldovich(tpm_ctx *ctx, uint64_t seed, double k_c, double disp_rms, double mom_factor, double *pos,
+                  double *mom) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(pos != nullptr, "null positions pointer");
+        TPM_REQUIRE(c.m <= c.max_n, "context capacity must hold mesh_n^3 particles");
+        double *d_mom = nullptr;
+        TPM_HIP(hipMalloc(&d_mom, 3 * c.m * sizeof(double)));
+        try {
+            tpm::zeldovich(c, seed, k_c, disp_rms, mom_factor, c.pos, d_mom);
+            TPM_HIP(hipMemcpyAsync(pos, c.pos, 3 * c.m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            if (mom) TPM_HIP(hipMemcpyAsync(mom, d_mom, 3 * c.m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            TPM_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            (void)hipFree(d_mom);
+            throw;
+        }
+        (void)hipFree(d_mom);
+        return TPM_OK;
+    });
+}
+
+}  // extern "C"

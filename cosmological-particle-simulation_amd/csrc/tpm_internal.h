@@ -1,0 +1,172 @@
+// Internal declarations of libtpm: the context, device helpers shared by the kernel files,
+// and the launchers each kernel file exports to the C ABI layer (tpm_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tpm.h"
+
+namespace tpm {
+
+// Fixed-point scale of the deterministic mass grid: one unit of mass is 2^32 counts.
+constexpr int kFixedShift = 32;
+// Morton key resolution per axis (3*21 = 63-bit keys); also the deepest octree level.
+constexpr int kKeyBits = 21;
+// Octree leaf size and depth limit (src/tree.rs:12-13 uses 8 and 40; the key resolution
+// bounds the depth here at 21 levels, below which particles share a leaf).
+constexpr int kLeafSize = 8;
+constexpr int kMaxDepth = kKeyBits;
+// Particles per walk group: one wavefront.
+constexpr int kGroup = 64;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define TPM_HIP(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::tpm::Error{TPM_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define TPM_REQUIRE(cond, msg)                                                             \
+    do {                                                                                   \
+        if (!(cond)) throw ::tpm::Error{TPM_E_INVALID, (msg)};                             \
+    } while (0)
+
+// Plan of one line FFT length: Stockham stages and the forward twiddle table
+// w[t] = exp(-2 pi i t / n), computed in double and stored in both precisions.
+struct FftPlan {
+    int n = 0;
+    int nst = 0;
+    int radix[24] = {0};
+};
+
+// Octree in structure-of-arrays form; nodes are stored level by level (breadth first) and
+// the children of a node are contiguous. TreeView is the plain part passed to kernels.
+struct TreeView {
+    uint2 *range;    // [start, end) into the sorted particle order
+    uint4 *link;     // x: first child, y: child count, z: child octant mask, w: level | walk-leaf<<8
+    float4 *box;     // centre (FP32, the reference's arithmetic) and half side
+    double4 *com;    // centre of mass (FP64) and particle count
+};
+
+struct Tree {
+    size_t cap = 0;
+    size_t count = 0;
+    uint2 *range = nullptr;
+    uint4 *link = nullptr;
+    float4 *box = nullptr;
+    double4 *com = nullptr;
+    std::vector<size_t> level_begin;  // node index where each level starts (+ end sentinel)
+    TreeView view() const { return TreeView{range, link, box, com}; }
+};
+
+struct Ctx {
+    tpm_params p{};
+    size_t max_n = 0;
+    int n = 0;          // mesh cells per side
+    int hp = 0;         // n/2 + 1, the half-spectrum length along z
+    size_t m = 0;       // n^3
+    size_t hs = 0;      // n * n * hp
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::string err;
+    tpm_stats stats{};
+
+    FftPlan plan;
+    float2 *tw32 = nullptr;   // forward twiddles, FP32
+    double2 *tw64 = nullptr;  // forward twiddles, FP64
+    double *kvec = nullptr;   // wavenumber(i) for i < n (src/fft.rs:86-95), FP64
+
+    // Particle buffers (capacity max_n).
+    double *pos = nullptr;     // 3 max_n staging positions (caller order)
+    float *acc = nullptr;      // 3 max_n accelerations (caller order)
+    uint64_t *keys = nullptr, *keys_alt = nullptr;
+    uint32_t *idx = nullptr, *idx_alt = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    double *spos = nullptr;    // 3 max_n positions in Morton order
+    unsigned long long *counter = nullptr;  // interaction counter + scratch words
+    int *flag = nullptr;       // non-finite flag
+
+    // Mesh buffers.
+    unsigned long long *grid = nullptr;  // fixed-point mass grid, m words
+    float2 *specA = nullptr, *specB = nullptr, *specC = nullptr;  // half spectra, hs each
+    float *force = nullptr;    // gx, gy, gz grids (3 m)
+    float *delta = nullptr;    // density contrast grid (m)
+
+    // FP64 judge buffers (allocated on first use).
+    double2 *dspecA = nullptr, *dspecB = nullptr, *dspecC = nullptr;
+    double *dforce = nullptr;
+    uint32_t *inv = nullptr;      // inverse of the Morton permutation
+    void *judge_buf = nullptr;    // judge targets and FP64 outputs
+    size_t judge_cap = 0;
+
+    Tree tree;
+    uint32_t *scan_tmp = nullptr;  // per-level child counts / offsets
+    size_t scan_cap = 0;
+    void *scan_tmp_storage = nullptr;
+    size_t scan_tmp_bytes = 0;
+};
+
+// ---------------------------------------------------------------------------------------
+// Launchers (all enqueue on ctx.stream).
+
+// Morton keys of positions (FP64, caller order) and a stable radix sort of (key, index);
+// then spos = positions gathered into key order.
+void sort_particles(Ctx &c, const double *d_pos, size_t n);
+
+// Fixed-point CIC deposit of sorted positions into c.grid. f32_weights: compute the stencil
+// in FP32 from the positions rounded to float (the reference's arithmetic).
+void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double *pos_src);
+// grid -> density contrast (FP32, the reference's c/mean - 1) and/or mass grid.
+void grid_to_float(Ctx &c, size_t n_particles, float *delta_out, float *mass_out);
+
+// PM force grids from the fixed-point grid (forward r2c fused with the density contrast,
+// Green's function, gradient and inverse passes). Writes c.force (gx, gy, gz).
+void pm_from_grid(Ctx &c, size_t n_particles, double a, double r_split, float *delta_out);
+// PM force grids from a density-contrast grid given on the device.
+void pm_from_delta(Ctx &c, const float *d_delta, double a, double r_split);
+// FP64 variant for the judge: writes c.dforce.
+void pm_from_grid_f64(Ctx &c, size_t n_particles, double a, double r_split);
+
+// CIC interpolation of the three force grids at sorted positions; writes acc[idx[i]].
+void interp_force(Ctx &c, size_t n, bool f32_weights, float *d_acc);
+// Interpolation of one grid at caller-order positions.
+void interp_one(Ctx &c, const float *d_cells, const double *d_pos, size_t n, bool f32_weights, float *d_out);
+
+// Octree over the sorted particles (levels, ranges, boxes, centres of mass).
+void build_tree(Ctx &c, size_t n);
+
+// Short-range force of every particle added into acc[idx[i]] (group walk or reference walk).
+void walk_short_range(Ctx &c, size_t n, double k, float *d_acc);
+// FP64 reference walk for listed targets (caller indices); writes 3 doubles per target.
+void walk_judge_f64(Ctx &c, size_t n, const uint32_t *d_targets_sorted, size_t m, double k, double theta,
+                    double *d_out);
+
+// FP64 interpolation of c.dforce at listed sorted indices.
+void interp_judge_f64(Ctx &c, const uint32_t *d_targets_sorted, size_t m, double *d_out);
+
+// Misc.
+void convert_f32_to_f64(Ctx &c, const float *src, double *dst, size_t count);
+void check_finite(Ctx &c, const float *d_acc, size_t count);
+
+// Zel'dovich displacement grids from device white noise (n^3 floats): c.dforce.
+void displacement_from_noise(Ctx &c, const float *d_noise, double kc);
+
+// Zel'dovich lattice ICs (ic_kernels.hip): positions and momenta (3 n^3 doubles each, lattice
+// order) on the device.
+void zeldovich(Ctx &c, uint64_t seed, double kc, double disp_rms, double mom_factor, double *d_pos, double *d_mom);
+
+// Plans and tables (fft_kernels.hip).
+void make_plan(Ctx &c);
+
+}  // namespace tpm

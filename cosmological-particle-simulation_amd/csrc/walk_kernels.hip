@@ -1,0 +1,530 @@
+// Short-range TreePM force: the Barnes-Hut walk with the Gaussian-split kernel.
+//
+// Replaces Tree::accel_on, pair and short_range_factor (src/tree.rs:54-66, 253-310).
+//
+// Group walk (default). One wavefront owns 64 consecutive particles in Morton order. The
+// wave walks the octree once for the whole group, against the group's bounding box:
+//   - a node is skipped when even the nearest point of the box is beyond r_cut + sqrt(3) half
+//     (the reference's per-particle cube test, src/tree.rs:274-278, made conservative);
+//   - a walk-leaf contributes its particle range (direct sums);
+//   - a split node is taken as a monopole only when 2 half < theta * d for the nearest point
+//     of the box, so the reference's criterion (src/tree.rs:297) holds for every particle of
+//     the group; otherwise its children are pushed.
+// The lists are then evaluated in LDS tiles of 64 sources, each lane applying the reference's
+// per-particle tests (pair cutoff r^2 <= r_cut^2, node cube test) and pair force. Where the
+// reference would accept a node for one particle but the group opens it, the particle gets
+// the node's children instead: the result is never less accurate than the reference's.
+//
+// Pair force (src/tree.rs:61-66): a = -K m S(r) d / (r^2 + eps^2)^{3/2},
+// S(r) = erfc(u) + (2/sqrt(pi)) u exp(-u^2), u = r / (2 r_s). In the FP32 walk, erfc(u) is
+// the Abramowitz & Stegun 7.1.26 rational approximation (Hastings),
+//   erfc(u) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-u^2),  t = 1 / (1 + p u),
+// with absolute error below 1.5e-7, which shares exp(-u^2) with the second term of S.
+//
+// Reference walk (TPM_FLAG_REF_WALK, and the FP64 judge): one thread per particle walking the
+// same octree with the reference's LIFO order and per-particle tests, in T = float or double.
+#include <cmath>
+
+#include "tpm_internal.h"
+
+namespace tpm {
+
+struct WalkArgs {
+    const double *spos;
+    const uint32_t *idx;
+    size_t n;
+    TreeView t;
+    double box;
+    double rcut;
+    double theta2;
+    float boxf;
+    float cut2;
+    float eps2;
+    float inv2rs;
+    float negk;
+    float *acc;
+    unsigned long long *counter;
+};
+
+constexpr float kErfP = 0.3275911f;
+constexpr float kErfA1 = 0.254829592f;
+constexpr float kErfA2 = -0.284496736f;
+constexpr float kErfA3 = 1.421413741f;
+constexpr float kErfA4 = -1.453152027f;
+constexpr float kErfA5 = 1.061405429f;
+constexpr float kTwoOverSqrtPi = 1.12837916709551257f;
+constexpr float kLog2e = 1.44269504088896341f;
+
+// S(r) for r = sqrt(r2), u = r * inv2rs.
+__device__ __forceinline__ float short_range_factor_f32(float r, float inv2rs) {
+    const float u = r * inv2rs;
+    const float e = exp2f(-(u * u) * kLog2e);
+    const float t = __frcp_rn(fmaf(kErfP, u, 1.0f));
+    const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, kErfA5, kErfA4), kErfA3), kErfA2), kErfA1);
+    return e * fmaf(kTwoOverSqrtPi, u, poly);
+}
+
+__device__ __forceinline__ void pair_f32(float dx, float dy, float dz, float r2, float mass, const WalkArgs &w,
+                                         float &ax, float &ay, float &az) {
+    const float s = r2 + w.eps2;
+    const float rs = __frsqrt_rn(s);
+    const float inv3 = rs * rs * rs;
+    const float r = __fsqrt_rn(r2);
+    const float f = w.negk * mass * short_range_factor_f32(r, w.inv2rs) * inv3;
+    ax = fmaf(dx, f, ax);
+    ay = fmaf(dy, f, ay);
+    az = fmaf(dz, f, az);
+}
+
+__device__ __forceinline__ int wave_excl_scan(int v, int &total) {
+    const int lane = threadIdx.x & 63;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmin(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ double min_image64(double d, double box) { return d - box * rint(d / box); }
+
+// Squared distance from point offset d (already minimum-imaged, relative to the box centre)
+// to an axis-aligned box of half extents h.
+__device__ __forceinline__ double box_dist2(double dx, double dy, double dz, double hx, double hy, double hz) {
+    const double ex = fmax(fabs(dx) - hx, 0.0);
+    const double ey = fmax(fabs(dy) - hy, 0.0);
+    const double ez = fmax(fabs(dz) - hz, 0.0);
+    return ex * ex + ey * ey + ez * ez;
+}
+
+constexpr int kStackCap = 768;
+constexpr int kStackAlloc = kStackCap + 192;
+constexpr int kRangeCap = 128;
+constexpr int kNodeCap = 128;
+
+struct WalkShared {
+    uint32_t stack[kStackAlloc];
+    uint32_t r_start[kRangeCap];
+    uint32_t r_off[kRangeCap + 1];
+    uint32_t nlist[kNodeCap];
+    float4 tileA[64];
+    float4 tileB[64];
+};
+
+template <bool WRAP>
+__device__ __forceinline__ void eval_particles(WalkShared &sh, int nr, int tot, const WalkArgs &w, double gx,
+                                               double gy, double gz, uint32_t self, float xi, float yi, float zi,
+                                               bool active, float &ax, float &ay, float &az, uint32_t &ninter) {
+    const int lane = threadIdx.x;
+    if (lane == 0) sh.r_off[nr] = (uint32_t)tot;
+    __syncthreads();
+    for (int base = 0; base < tot; base += 64) {
+        const int e = base + lane;
+        if (e < tot) {
+            int lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((int)sh.r_off[mid] <= e)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const uint32_t j = sh.r_start[lo] + (uint32_t)(e - (int)sh.r_off[lo]);
+            double dx = w.spos[3 * (size_t)j] - gx;
+            double dy = w.spos[3 * (size_t)j + 1] - gy;
+            double dz = w.spos[3 * (size_t)j + 2] - gz;
+            if (!WRAP) {
+                dx = min_image64(dx, w.box);
+                dy = min_image64(dy, w.box);
+                dz = min_image64(dz, w.box);
+            }
+            sh.tileA[lane] = make_float4((float)dx, (float)dy, (float)dz, __uint_as_float(j));
+        }
+        __syncthreads();
+        const int cnt = min(64, tot - base);
+        if (active) {
+            for (int s = 0; s < cnt; ++s) {
+                const float4 src = sh.tileA[s];
+                float dx = xi - src.x;
+                float dy = yi - src.y;
+                float dz = zi - src.z;
+                if (WRAP) {
+                    dx -= w.boxf * rintf(dx / w.boxf);
+                    dy -= w.boxf * rintf(dy / w.boxf);
+                    dz -= w.boxf * rintf(dz / w.boxf);
+                }
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                if (r2 <= w.cut2 && __float_as_uint(src.w) != self) {
+                    pair_f32(dx, dy, dz, r2, 1.0f, w, ax, ay, az);
+                    ++ninter;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <bool WRAP>
+__device__ __forceinline__ void eval_nodes(WalkShared &sh, int nn, const WalkArgs &w, double gx, double gy, double gz,
+                                           float xi, float yi, float zi, bool active, float &ax, float &ay,
+                                           float &az, uint32_t &ninter) {
+    const int lane = threadIdx.x;
+    const float sqrt3 = 1.7320508075688772f;
+    for (int base = 0; base < nn; base += 64) {
+        const int e = base + lane;
+        if (e < nn) {
+            const uint32_t q = sh.nlist[e];
+            const double4 cm = w.t.com[q];
+            const float4 bx = w.t.box[q];
+            double cx = cm.x - gx, cy = cm.y - gy, cz = cm.z - gz;
+            double bxx = (double)bx.x - gx, byy = (double)bx.y - gy, bzz = (double)bx.z - gz;
+            if (!WRAP) {
+                cx = min_image64(cx, w.box);
+                cy = min_image64(cy, w.box);
+                cz = min_image64(cz, w.box);
+                bxx = min_image64(bxx, w.box);
+                byy = min_image64(byy, w.box);
+                bzz = min_image64(bzz, w.box);
+            }
+            // src/tree.rs:275: max_r = r_cut + SQRT_3 * half, in FP32.
+            const float max_r = (float)w.rcut + sqrt3 * bx.w;
+            sh.tileA[lane] = make_float4((float)cx, (float)cy, (float)cz, (float)cm.w);
+            sh.tileB[lane] = make_float4((float)bxx, (float)byy, (float)bzz, max_r * max_r);
+        }
+        __syncthreads();
+        const int cnt = min(64, nn - base);
+        if (active) {
+            for (int s = 0; s < cnt; ++s) {
+                const float4 bb = sh.tileB[s];
+                float ux = bb.x - xi, uy = bb.y - yi, uz = bb.z - zi;
+                if (WRAP) {
+                    ux -= w.boxf * rintf(ux / w.boxf);
+                    uy -= w.boxf * rintf(uy / w.boxf);
+                    uz -= w.boxf * rintf(uz / w.boxf);
+                }
+                if (fmaf(ux, ux, fmaf(uy, uy, uz * uz)) > bb.w) continue;
+                const float4 src = sh.tileA[s];
+                float dx = xi - src.x, dy = yi - src.y, dz = zi - src.z;
+                if (WRAP) {
+                    dx -= w.boxf * rintf(dx / w.boxf);
+                    dy -= w.boxf * rintf(dy / w.boxf);
+                    dz -= w.boxf * rintf(dz / w.boxf);
+                }
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                pair_f32(dx, dy, dz, r2, src.w, w, ax, ay, az);
+                ++ninter;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <bool WRAP>
+__device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, size_t g0, double gx, double gy, double gz,
+                                double hx, double hy, double hz, uint32_t self, float xi, float yi, float zi,
+                                bool active, float &ax, float &ay, float &az, uint32_t &ninter) {
+    const int lane = threadIdx.x;
+    const double sqrt3 = 1.7320508075688772;
+    int size = 1;
+    if (lane == 0) sh.stack[0] = 0;
+    int nr = 0, tot = 0, nn = 0;
+    __syncthreads();
+    while (size > 0) {
+        int room = (kStackCap - size) / 7;
+        int k = min(64, size);
+        if (k > room) k = room > 1 ? room : 1;
+        uint32_t node = 0xFFFFFFFFu;
+        if (lane < k) node = sh.stack[size - k + lane];
+        __syncthreads();
+        size -= k;
+        int kind = 0;  // 0 none, 1 leaf range, 2 accept, 3 open
+        uint2 rng = make_uint2(0, 0);
+        uint4 lk = make_uint4(0, 0, 0, 0);
+        if (node != 0xFFFFFFFFu) {
+            lk = w.t.link[node];
+            rng = w.t.range[node];
+            const float4 bx = w.t.box[node];
+            double dx = (double)bx.x - gx, dy = (double)bx.y - gy, dz = (double)bx.z - gz;
+            if (!WRAP) {
+                dx = min_image64(dx, w.box);
+                dy = min_image64(dy, w.box);
+                dz = min_image64(dz, w.box);
+            }
+            const double maxr = w.rcut + sqrt3 * (double)bx.w;
+            const double d2 = box_dist2(dx, dy, dz, hx, hy, hz);
+            if (WRAP || d2 <= maxr * maxr * (1.0 + 1e-6)) {
+                if (lk.w >> 8) {
+                    kind = 1;
+                } else {
+                    const double4 cm = w.t.com[node];
+                    double cx = cm.x - gx, cy = cm.y - gy, cz = cm.z - gz;
+                    if (!WRAP) {
+                        cx = min_image64(cx, w.box);
+                        cy = min_image64(cy, w.box);
+                        cz = min_image64(cz, w.box);
+                    }
+                    const double dc2 = WRAP ? 0.0 : box_dist2(cx, cy, cz, hx, hy, hz);
+                    const double side = 2.0 * (double)bx.w;
+                    kind = (side * side < w.theta2 * dc2 * (1.0 - 1e-6)) ? 2 : 3;
+                }
+            }
+        }
+        // Appends (wave-wide prefix sums).
+        int t_open, t_leaf, t_leafcnt, t_acc;
+        const int o_open = wave_excl_scan(kind == 3 ? (int)lk.y : 0, t_open);
+        const int o_leaf = wave_excl_scan(kind == 1 ? 1 : 0, t_leaf);
+        const int o_leafcnt = wave_excl_scan(kind == 1 ? (int)(rng.y - rng.x) : 0, t_leafcnt);
+        const int o_acc = wave_excl_scan(kind == 2 ? 1 : 0, t_acc);
+        if (kind == 3)
+            for (uint32_t cix = 0; cix < lk.y; ++cix) {
+                const int at = size + o_open + (int)cix;
+                if (at < kStackAlloc) sh.stack[at] = lk.x + cix;
+            }
+        if (kind == 1) {
+            sh.r_start[nr + o_leaf] = rng.x;
+            sh.r_off[nr + o_leaf] = (uint32_t)(tot + o_leafcnt);
+        }
+        if (kind == 2) sh.nlist[nn + o_acc] = node;
+        size += t_open;
+        if (size > kStackAlloc) {
+            // Cannot happen with the pop limit above (bounded by kStackCap + 7 * depth); recorded
+            // as an error instead of writing past the stack.
+            if (lane == 0 && w.counter) atomicAdd(w.counter + 1, 1ull);
+            size = kStackAlloc;
+        }
+        nr += t_leaf;
+        tot += t_leafcnt;
+        nn += t_acc;
+        __syncthreads();
+        if (nr > kRangeCap - 64) {
+            eval_particles<WRAP>(sh, nr, tot, w, gx, gy, gz, self, xi, yi, zi, active, ax, ay, az, ninter);
+            nr = 0;
+            tot = 0;
+        }
+        if (nn > kNodeCap - 64) {
+            eval_nodes<WRAP>(sh, nn, w, gx, gy, gz, xi, yi, zi, active, ax, ay, az, ninter);
+            nn = 0;
+        }
+    }
+    if (nr > 0) eval_particles<WRAP>(sh, nr, tot, w, gx, gy, gz, self, xi, yi, zi, active, ax, ay, az, ninter);
+    if (nn > 0) eval_nodes<WRAP>(sh, nn, w, gx, gy, gz, xi, yi, zi, active, ax, ay, az, ninter);
+}
+
+__global__ void __launch_bounds__(64) k_group_walk(WalkArgs w) {
+    __shared__ WalkShared sh;
+    const int lane = threadIdx.x;
+    const size_t g0 = (size_t)blockIdx.x * kGroup;
+    const size_t i = g0 + lane;
+    const bool active = i < w.n;
+    const size_t ii = active ? i : g0;
+    const double px = w.spos[3 * ii], py = w.spos[3 * ii + 1], pz = w.spos[3 * ii + 2];
+    const double mnx = wave_min(px), mny = wave_min(py), mnz = wave_min(pz);
+    const double mxx = wave_max(px), mxy = wave_max(py), mxz = wave_max(pz);
+    const double gx = 0.5 * (mnx + mxx), gy = 0.5 * (mny + mxy), gz = 0.5 * (mnz + mxz);
+    // Half extents, widened by a relative 1e-12 of the box so the box surely contains the group.
+    const double pad = 1e-12 * w.box;
+    const double hx = 0.5 * (mxx - mnx) + pad, hy = 0.5 * (mxy - mny) + pad, hz = 0.5 * (mxz - mnz) + pad;
+    const float xi = (float)(px - gx), yi = (float)(py - gy), zi = (float)(pz - gz);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    uint32_t ninter = 0;
+    // Sources are placed at their image nearest the group centre; that image is the one nearest
+    // each particle whenever half extent + r_cut < L/2 on every axis. Otherwise every pair and
+    // node distance is wrapped individually.
+    const double lim = 0.5 * w.box * (1.0 - 1e-9);
+    const bool wrap = (hx + w.rcut >= lim) || (hy + w.rcut >= lim) || (hz + w.rcut >= lim);
+    if (wrap)
+        group_walk_body<true>(sh, w, g0, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay, az, ninter);
+    else
+        group_walk_body<false>(sh, w, g0, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay, az,
+                               ninter);
+    if (active) {
+        const size_t o = w.idx[i];
+        w.acc[3 * o] += ax;
+        w.acc[3 * o + 1] += ay;
+        w.acc[3 * o + 2] += az;
+    }
+    int tot;
+    wave_excl_scan(active ? (int)ninter : 0, tot);
+    if (lane == 0 && w.counter) atomicAdd(w.counter, (unsigned long long)tot);
+}
+
+// -------------------------------------------------------------------- reference walk
+
+template <typename T>
+struct RefWalkArgs {
+    const double *spos;
+    const uint32_t *idx;
+    const uint32_t *targets;  // sorted indices to walk, or null = all particles in order
+    size_t m;
+    TreeView t;
+    T box, r_split, rcut, eps, theta, k;
+    T *out_f64;    // 3 per target (judge), or null
+    float *acc;    // += into caller order, or null
+    unsigned long long *counter;
+};
+
+template <typename T>
+__device__ __forceinline__ T sr_factor_ref(T r, T r_split) {
+    const T u = r / ((T)2 * r_split);
+    return erfc(u) + (T)1.12837916709551257 * u * exp(-u * u);
+}
+
+template <typename T>
+__device__ __forceinline__ void min_image_ref(T l, const T a[3], const T b[3], T d[3]) {
+    for (int q = 0; q < 3; ++q) {
+        const T x = a[q] - b[q];
+        d[q] = x - l * round(x / l);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void pair_ref(const T d[3], T mass, const RefWalkArgs<T> &w, T acc[3]) {
+    const T r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const T s = r2 + w.eps * w.eps;
+    const T f = -w.k * mass * sr_factor_ref<T>(sqrt(r2), w.r_split) / (s * sqrt(s));
+    for (int q = 0; q < 3; ++q) acc[q] += d[q] * f;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(128) k_ref_walk(RefWalkArgs<T> w) {
+    const size_t tix = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tix >= w.m) return;
+    const size_t i = w.targets ? w.targets[tix] : tix;
+    const T xi[3] = {(T)w.spos[3 * i], (T)w.spos[3 * i + 1], (T)w.spos[3 * i + 2]};
+    const T cut2 = w.rcut * w.rcut;
+    const T sqrt3 = (T)1.7320508075688772;
+    T acc[3] = {0, 0, 0};
+    uint32_t stack[8 * (kMaxDepth + 2)];
+    int sp = 0;
+    stack[sp++] = 0;
+    unsigned long long cnt = 0;
+    while (sp > 0) {
+        const uint32_t nd = stack[--sp];
+        const float4 bx = w.t.box[nd];
+        const uint4 lk = w.t.link[nd];
+        const T ctr[3] = {(T)bx.x, (T)bx.y, (T)bx.z};
+        T tc[3];
+        min_image_ref<T>(w.box, ctr, xi, tc);
+        const T max_r = w.rcut + sqrt3 * (T)bx.w;
+        if (tc[0] * tc[0] + tc[1] * tc[1] + tc[2] * tc[2] > max_r * max_r) continue;
+        if (lk.w >> 8) {
+            const uint2 r = w.t.range[nd];
+            for (uint32_t j = r.x; j < r.y; ++j) {
+                if (j == i) continue;
+                const T xj[3] = {(T)w.spos[3 * (size_t)j], (T)w.spos[3 * (size_t)j + 1], (T)w.spos[3 * (size_t)j + 2]};
+                T d[3];
+                min_image_ref<T>(w.box, xi, xj, d);
+                if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] > cut2) continue;
+                pair_ref<T>(d, (T)1, w, acc);
+                ++cnt;
+            }
+        } else {
+            const double4 cm = w.t.com[nd];
+            const T com[3] = {(T)cm.x, (T)cm.y, (T)cm.z};
+            T tcm[3];
+            min_image_ref<T>(w.box, com, xi, tcm);
+            const T r = sqrt(tcm[0] * tcm[0] + tcm[1] * tcm[1] + tcm[2] * tcm[2]);
+            if ((T)2 * (T)bx.w < w.theta * r) {
+                const T d[3] = {-tcm[0], -tcm[1], -tcm[2]};
+                pair_ref<T>(d, (T)cm.w, w, acc);
+                ++cnt;
+            } else {
+                for (uint32_t cix = 0; cix < lk.y; ++cix) stack[sp++] = lk.x + cix;
+            }
+        }
+    }
+    if (w.out_f64) {
+        w.out_f64[3 * tix] = acc[0];
+        w.out_f64[3 * tix + 1] = acc[1];
+        w.out_f64[3 * tix + 2] = acc[2];
+    }
+    if (w.acc) {
+        const size_t o = w.idx[i];
+        w.acc[3 * o] += (float)acc[0];
+        w.acc[3 * o + 1] += (float)acc[1];
+        w.acc[3 * o + 2] += (float)acc[2];
+    }
+    if (w.counter) atomicAdd(w.counter, cnt);
+}
+
+static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+void walk_short_range(Ctx &c, size_t n, double k, float *d_acc) {
+    if (n == 0) return;
+    if (c.p.flags & TPM_FLAG_REF_WALK) {
+        RefWalkArgs<float> w{};
+        w.spos = c.spos;
+        w.idx = c.idx;
+        w.targets = nullptr;
+        w.m = n;
+        w.t = c.tree.view();
+        w.box = (float)c.p.box_size;
+        w.r_split = (float)c.p.r_split;
+        w.rcut = (float)c.p.r_cut;
+        w.eps = (float)c.p.softening;
+        w.theta = (float)c.p.theta;
+        w.k = (float)k;
+        w.out_f64 = nullptr;
+        w.acc = d_acc;
+        w.counter = c.counter;
+        hipLaunchKernelGGL(k_ref_walk<float>, dim3(blocks(n, 128)), dim3(128), 0, c.stream, w);
+        TPM_HIP(hipGetLastError());
+        return;
+    }
+    WalkArgs w{};
+    w.spos = c.spos;
+    w.idx = c.idx;
+    w.n = n;
+    w.t = c.tree.view();
+    w.box = c.p.box_size;
+    w.rcut = c.p.r_cut;
+    w.theta2 = c.p.theta * c.p.theta;
+    w.boxf = (float)c.p.box_size;
+    w.cut2 = (float)c.p.r_cut * (float)c.p.r_cut;
+    w.eps2 = (float)c.p.softening * (float)c.p.softening;
+    w.inv2rs = (float)(1.0 / (2.0 * c.p.r_split));
+    w.negk = (float)(-k);
+    w.acc = d_acc;
+    w.counter = c.counter;
+    hipLaunchKernelGGL(k_group_walk, dim3(blocks(n, kGroup)), dim3(kGroup), 0, c.stream, w);
+    TPM_HIP(hipGetLastError());
+}
+
+void walk_judge_f64(Ctx &c, size_t n, const uint32_t *d_targets_sorted, size_t m, double k, double theta,
+                    double *d_out) {
+    (void)n;
+    if (m == 0) return;
+    RefWalkArgs<double> w{};
+    w.spos = c.spos;
+    w.idx = c.idx;
+    w.targets = d_targets_sorted;
+    w.m = m;
+    w.t = c.tree.view();
+    w.box = c.p.box_size;
+    w.r_split = c.p.r_split;
+    w.rcut = c.p.r_cut;
+    w.eps = c.p.softening;
+    w.theta = theta;
+    w.k = k;
+    w.out_f64 = d_out;
+    w.acc = nullptr;
+    w.counter = nullptr;
+    hipLaunchKernelGGL(k_ref_walk<double>, dim3(blocks(m, 128)), dim3(128), 0, c.stream, w);
+    TPM_HIP(hipGetLastError());
+}
+
+}  // namespace tpm

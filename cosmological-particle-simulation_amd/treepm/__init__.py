@@ -1,0 +1,302 @@
+"""Python binding of libtpm (the C ABI in include/tpm.h), used by the tests and the benchmark.
+
+It mirrors the reference's interface for the force path (src/integrator.rs:140-161 and the
+public functions under it) so that the tests read like the reference's own:
+
+    Solver(params, max_particles)            PoissonSolver::new + the tree (src/poisson.rs:23)
+    Solver.accelerations(pos, a)             Simulation::accelerations (src/integrator.rs:140)
+    Solver.deposit(pos, mass)                Mesh::deposit (src/mesh.rs:62)
+    Solver.density_contrast(pos)             Simulation::density_contrast (src/integrator.rs:117)
+    Solver.pm_accel(delta, a, r_split)       PoissonSolver::acceleration (src/poisson.rs:33)
+    Solver.interpolate(cells, pos)           Mesh::interpolate (src/mesh.rs:72)
+    Solver.short_range(pos, k)               tree::accelerations (src/tree.rs:314)
+    force_split(...)                         ForceSplit::with (src/tree.rs:40)
+
+Every call goes through the HIP library; there is no CPU fallback. Loading fails loudly when
+libtpm.so has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libtpm.so")
+
+TPM_OK = 0
+TPM_E_INVALID = -1
+TPM_E_DEVICE = -2
+TPM_E_CAPACITY = -3
+TPM_E_NONFINITE = -4
+
+FLAG_TREEPM = 1
+FLAG_REF_LEAF_RULE = 2
+FLAG_CHECK_FINITE = 4
+FLAG_REF_WALK = 8
+
+# Symbols the header declares (checked by the CPU test suite against the built library).
+EXPORTS = [
+    "tpm_version", "tpm_create", "tpm_destroy", "tpm_last_error", "tpm_set_params",
+    "tpm_accelerations_f32", "tpm_accelerations_f64", "tpm_accelerations_dev_f64",
+    "tpm_get_stats", "tpm_set_timing", "tpm_deposit_f32", "tpm_deposit_f64",
+    "tpm_density_contrast_f32", "tpm_pm_accel", "tpm_interpolate_f32",
+    "tpm_short_range_f32", "tpm_short_range_f64", "tpm_judge_f64",
+    "tpm_zeldovich", "tpm_zeldovich_dev",
+]
+
+
+class TpmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tpm error {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("mesh_n", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
+        ("box_size", ctypes.c_double),
+        ("omega_m", ctypes.c_double),
+        ("r_split", ctypes.c_double),
+        ("r_cut", ctypes.c_double),
+        ("softening", ctypes.c_double),
+        ("theta", ctypes.c_double),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", ctypes.c_double),
+        ("ms_sort", ctypes.c_double),
+        ("ms_deposit", ctypes.c_double),
+        ("ms_fft", ctypes.c_double),
+        ("ms_interp", ctypes.c_double),
+        ("ms_tree", ctypes.c_double),
+        ("ms_walk", ctypes.c_double),
+        ("interactions", ctypes.c_uint64),
+        ("nodes", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load():
+    """Loads libtpm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libtpm.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    P, S, D, F = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_float
+    sigs = {
+        "tpm_version": (ctypes.c_char_p, []),
+        "tpm_create": (ctypes.c_int, [ctypes.POINTER(Params), S, ctypes.POINTER(ctypes.c_void_p)]),
+        "tpm_destroy": (None, [P]),
+        "tpm_last_error": (ctypes.c_char_p, [P]),
+        "tpm_set_params": (ctypes.c_int, [P, ctypes.POINTER(Params)]),
+        "tpm_accelerations_f32": (ctypes.c_int, [P, P, S, D, P, P]),
+        "tpm_accelerations_f64": (ctypes.c_int, [P, P, S, D, P, P]),
+        "tpm_accelerations_dev_f64": (ctypes.c_int, [P, P, S, D, P, P, P]),
+        "tpm_get_stats": (ctypes.c_int, [P, ctypes.POINTER(Stats)]),
+        "tpm_set_timing": (ctypes.c_int, [P, ctypes.c_int]),
+        "tpm_deposit_f32": (ctypes.c_int, [P, P, S, F, P]),
+        "tpm_deposit_f64": (ctypes.c_int, [P, P, S, D, P]),
+        "tpm_density_contrast_f32": (ctypes.c_int, [P, P, S, P]),
+        "tpm_pm_accel": (ctypes.c_int, [P, P, D, D, P, P, P]),
+        "tpm_interpolate_f32": (ctypes.c_int, [P, P, P, S, P]),
+        "tpm_short_range_f32": (ctypes.c_int, [P, P, S, D, P]),
+        "tpm_short_range_f64": (ctypes.c_int, [P, P, S, D, P]),
+        "tpm_judge_f64": (ctypes.c_int, [P, P, S, P, S, D, D, P]),
+        "tpm_zeldovich": (ctypes.c_int, [P, ctypes.c_uint64, D, D, D, P, P]),
+        "tpm_zeldovich_dev": (ctypes.c_int, [P, ctypes.c_uint64, D, D, D, P, P, P]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.reshape(shape) if shape is not None else a
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(shape) if shape is not None else a
+
+
+def force_split(box_size, mesh_n, particles_per_side, softening_scale, theta):
+    """ForceSplit::with (src/tree.rs:40-50): r_s = 1.25 cells, r_cut = 4.5 r_s, FP32 like the reference."""
+    f = np.float32
+    cell = f(box_size) / f(mesh_n)
+    r_split = f(1.25) * cell
+    spacing = f(box_size) / f(particles_per_side)
+    return {"r_split": float(r_split), "r_cut": float(f(4.5) * r_split),
+            "softening": float(f(softening_scale) * spacing), "theta": float(theta)}
+
+
+class Solver:
+    """One libtpm context: mesh, FFT tables, tree scratch and device buffers."""
+
+    def __init__(self, mesh_n, box_size, omega_m=0.315, split=None, max_particles=1 << 16, flags=0):
+        self.lib = load()
+        p = Params()
+        p.mesh_n = int(mesh_n)
+        p.box_size = float(box_size)
+        p.omega_m = float(omega_m)
+        fl = int(flags)
+        if split is not None:
+            fl |= FLAG_TREEPM
+            p.r_split = float(split["r_split"])
+            p.r_cut = float(split["r_cut"])
+            p.softening = float(split["softening"])
+            p.theta = float(split["theta"])
+        p.flags = fl
+        self.params = p
+        self.n = int(mesh_n)
+        self.max_particles = int(max_particles)
+        h = ctypes.c_void_p()
+        rc = self.lib.tpm_create(ctypes.byref(p), self.max_particles, ctypes.byref(h))
+        if rc != TPM_OK:
+            raise TpmError(rc, self.lib.tpm_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tpm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != TPM_OK:
+            raise TpmError(rc, self.lib.tpm_last_error(self.h).decode())
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        self._check(self.lib.tpm_set_params(self.h, ctypes.byref(self.params)))
+
+    def set_timing(self, on=True):
+        self._check(self.lib.tpm_set_timing(self.h, 1 if on else 0))
+
+    def stats(self):
+        s = Stats()
+        self._check(self.lib.tpm_get_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def accelerations(self, pos, a, want_delta=False):
+        """Simulation::accelerations: returns (acc (N,3) float32, delta or None)."""
+        if np.asarray(pos).dtype == np.float64:
+            pos = _f64(pos, (-1, 3))
+            fn = self.lib.tpm_accelerations_f64
+        else:
+            pos = _f32(pos, (-1, 3))
+            fn = self.lib.tpm_accelerations_f32
+        acc = np.empty((len(pos), 3), np.float32)
+        delta = np.empty(self.n ** 3, np.float32) if want_delta else None
+        self._check(fn(self.h, _ptr(pos), len(pos), float(a), _ptr(acc), _ptr(delta)))
+        return acc, delta
+
+    def accelerations_dev(self, d_pos, n, a, d_acc, d_delta=0, stream=0):
+        """Device-pointer form (integers are device addresses, e.g. torch's data_ptr())."""
+        self._check(self.lib.tpm_accelerations_dev_f64(self.h, ctypes.c_void_p(d_pos), int(n), float(a),
+                                                       ctypes.c_void_p(d_acc), ctypes.c_void_p(d_delta or None),
+                                                       ctypes.c_void_p(stream or None)))
+
+    def deposit(self, pos, mass=1.0):
+        """Mesh::deposit of `mass` per particle; returns the n^3 grid (float32)."""
+        out = np.empty(self.n ** 3, np.float32)
+        if np.asarray(pos).dtype == np.float64:
+            pos = _f64(pos, (-1, 3))
+            self._check(self.lib.tpm_deposit_f64(self.h, _ptr(pos), len(pos), float(mass), _ptr(out)))
+        else:
+            pos = _f32(pos, (-1, 3))
+            self._check(self.lib.tpm_deposit_f32(self.h, _ptr(pos), len(pos), float(mass), _ptr(out)))
+        return out
+
+    def density_contrast(self, pos):
+        pos = _f32(pos, (-1, 3))
+        out = np.empty(self.n ** 3, np.float32)
+        self._check(self.lib.tpm_density_contrast_f32(self.h, _ptr(pos), len(pos), _ptr(out)))
+        return out
+
+    def pm_accel(self, delta, a, r_split=0.0):
+        """PoissonSolver::acceleration: returns (3, n^3) float32 grids."""
+        delta = _f32(delta).reshape(-1)
+        assert delta.size == self.n ** 3, "mesh size must match the solver"
+        g = np.empty((3, self.n ** 3), np.float32)
+        self._check(self.lib.tpm_pm_accel(self.h, _ptr(delta), float(a), float(r_split),
+                                          _ptr(g[0]), _ptr(g[1]), _ptr(g[2])))
+        return g
+
+    def interpolate(self, cells, pos):
+        cells = _f32(cells).reshape(-1)
+        pos = _f32(pos, (-1, 3))
+        out = np.empty(len(pos), np.float32)
+        self._check(self.lib.tpm_interpolate_f32(self.h, _ptr(cells), _ptr(pos), len(pos), _ptr(out)))
+        return out
+
+    def short_range(self, pos, k):
+        """tree::accelerations (short-range force only) with coupling k."""
+        if np.asarray(pos).dtype == np.float64:
+            pos = _f64(pos, (-1, 3))
+            fn = self.lib.tpm_short_range_f64
+        else:
+            pos = _f32(pos, (-1, 3))
+            fn = self.lib.tpm_short_range_f32
+        acc = np.empty((len(pos), 3), np.float32)
+        self._check(fn(self.h, _ptr(pos), len(pos), float(k), _ptr(acc)))
+        return acc
+
+    def zeldovich(self, seed, k_c, disp_rms, mom_factor=0.0, want_momenta=False):
+        """Lattice Zel'dovich ICs (mesh_n^3 particles): returns (pos (N,3) f64, mom or None)."""
+        npart = self.n ** 3
+        pos = np.empty((npart, 3), np.float64)
+        mom = np.empty((npart, 3), np.float64) if want_momenta else None
+        self._check(self.lib.tpm_zeldovich(self.h, int(seed), float(k_c), float(disp_rms), float(mom_factor),
+                                           _ptr(pos), _ptr(mom)))
+        return pos, mom
+
+    def zeldovich_dev(self, seed, k_c, disp_rms, mom_factor, d_pos, d_mom=0, stream=0):
+        self._check(self.lib.tpm_zeldovich_dev(self.h, int(seed), float(k_c), float(disp_rms), float(mom_factor),
+                                               ctypes.c_void_p(d_pos), ctypes.c_void_p(d_mom or None),
+                                               ctypes.c_void_p(stream or None)))
+
+    def judge(self, pos, idx, a, theta=0.2):
+        """FP64 TreePM force (FP64 PM + FP64 per-particle walk at `theta`) at caller indices idx."""
+        pos = _f64(pos, (-1, 3))
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        out = np.empty((len(idx), 3), np.float64)
+        self._check(self.lib.tpm_judge_f64(self.h, _ptr(pos), len(pos), _ptr(idx), len(idx), float(a),
+                                           float(theta), _ptr(out)))
+        return out
+
+
+def coupling(omega_m, box_size, a, n):
+    """K = 1.5 Omega_m L^3 / (4 pi a N) in float32 (src/integrator.rs:150-152)."""
+    f = np.float32
+    l = f(box_size)
+    return float(f(1.5) * f(omega_m) * (l * l * l) / (f(4.0) * f(np.pi) * f(a) * f(n)))

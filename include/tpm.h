@@ -1,0 +1,158 @@
+/*
+ * tpm.h — C ABI of the MI355X TreePM force solver (libtpm.so).
+ *
+ * The reference (a Rust crate) has no FFI for this path; its "operator" is the method
+ * Simulation::accelerations (src/integrator.rs:140-161) and the public functions it is made
+ * of. Each entry point below names the reference function it replaces, so a Rust
+ * `extern "C"` block can bind it beside the CPU path (see INTEGRATION.md for the binding).
+ *
+ * Conventions
+ *  - Every call returns 0 on success and a negative TPM_E_* code on failure; nothing aborts
+ *    across the ABI. tpm_last_error() returns a message for the last failure on a context.
+ *  - Positions are AoS triples (x, y, z), the layout of Rust's Vec<[f32; 3]>; accelerations
+ *    are written in the same layout and in the caller's particle order. Grids are n^3 values
+ *    indexed (ix*n + iy)*n + iz, z fastest (src/mesh.rs:11,24-26).
+ *  - Units are the reference's: H0 = 1, lengths in the box's units, accelerations dp/dt.
+ *  - Host-pointer entry points copy in and out and return when the result is written.
+ *    *_dev entry points take device pointers and enqueue on the given HIP stream
+ *    (NULL = the context's stream) without synchronising.
+ *  - A context owns all device memory, FFT tables and tree scratch (the PoissonSolver
+ *    analogue, src/poisson.rs:23-29). Calls on one context must not overlap.
+ */
+#ifndef TPM_H
+#define TPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TPM_OK 0
+#define TPM_E_INVALID (-1)   /* bad argument (size, null pointer, mesh mismatch) */
+#define TPM_E_DEVICE (-2)    /* HIP runtime error */
+#define TPM_E_CAPACITY (-3)  /* more particles than the context was created for */
+#define TPM_E_NONFINITE (-4) /* non-finite acceleration detected (TPM_FLAG_CHECK_FINITE) */
+
+/* Flags for tpm_params.flags. */
+#define TPM_FLAG_TREEPM 1u           /* add the tree short-range force (split = Some(..)); off = pure PM */
+#define TPM_FLAG_REF_LEAF_RULE 2u    /* walk a node whose octant-0 child is empty as a leaf, like
+                                        the reference's children[0] test (src/tree.rs:280) */
+#define TPM_FLAG_CHECK_FINITE 4u     /* scan accelerations for NaN/Inf and fail with TPM_E_NONFINITE */
+#define TPM_FLAG_REF_WALK 8u         /* per-particle LIFO walk with the reference's per-particle opening
+                                        test (src/tree.rs:264-310) instead of the group walk */
+
+/*
+ * Solver parameters: PoissonSolver::new(n, L) (src/poisson.rs:23) plus ForceSplit
+ * (src/tree.rs:21-30) and the cosmology's Omega_m. r_split is used for the PM filter only
+ * when TPM_FLAG_TREEPM is set (src/integrator.rs:142).
+ */
+typedef struct tpm_params {
+    int32_t mesh_n;    /* PM cells per side, 2..1024 */
+    uint32_t flags;    /* TPM_FLAG_* */
+    double box_size;   /* periodic box side L */
+    double omega_m;    /* matter density parameter */
+    double r_split;    /* Gaussian split scale r_s */
+    double r_cut;      /* short-range cutoff */
+    double softening;  /* Plummer softening */
+    double theta;      /* opening angle */
+} tpm_params;
+
+typedef struct tpm_ctx tpm_ctx;
+
+/* Timing and work counters of the last force evaluation on a context. */
+typedef struct tpm_stats {
+    double ms_total;       /* device time of the whole evaluation */
+    double ms_sort;        /* Morton keys + sort + gather */
+    double ms_deposit;     /* fixed-point CIC deposit */
+    double ms_fft;         /* r2c, Green multiply, inverse passes */
+    double ms_interp;      /* CIC force interpolation */
+    double ms_tree;        /* octree build + centres of mass */
+    double ms_walk;        /* short-range walk */
+    uint64_t interactions; /* particle-particle + particle-node interactions evaluated */
+    uint64_t nodes;        /* octree nodes */
+} tpm_stats;
+
+/* Version string of the library. */
+const char *tpm_version(void);
+
+/* Creates a context for up to max_particles particles. Replaces Simulation::new's
+ * PoissonSolver::new (src/integrator.rs:67, src/poisson.rs:23-29). */
+int tpm_create(const tpm_params *params, size_t max_particles, tpm_ctx **out);
+
+/* Releases every device allocation of the context. */
+void tpm_destroy(tpm_ctx *ctx);
+
+/* Message for the last failed call on ctx (or for tpm_create when ctx is NULL). */
+const char *tpm_last_error(const tpm_ctx *ctx);
+
+/* Updates the parameters of an existing context (mesh_n must not change). */
+int tpm_set_params(tpm_ctx *ctx, const tpm_params *params);
+
+/*
+ * Full TreePM force evaluation: Simulation::accelerations (src/integrator.rs:140-161).
+ * pos: 3n positions in [0, L); acc: 3n accelerations out; delta: n_mesh^3 density contrast
+ * out (src/integrator.rs:117-126), or NULL. The f32 variant computes CIC weights in FP32 as
+ * the reference does; the f64 variant computes them in FP64. Both walk the tree in FP32 on
+ * positions kept in FP64.
+ */
+int tpm_accelerations_f32(tpm_ctx *ctx, const float *pos, size_t n, double a, float *acc, float *delta);
+int tpm_accelerations_f64(tpm_ctx *ctx, const double *pos, size_t n, double a, float *acc, float *delta);
+
+/* Device-pointer form of tpm_accelerations_f64, enqueued on `stream` (hipStream_t or NULL). */
+int tpm_accelerations_dev_f64(tpm_ctx *ctx, const double *d_pos, size_t n, double a, float *d_acc,
+                              float *d_delta, void *stream);
+
+/* Statistics of the last force evaluation (device timings need TPM_FLAG timing enabled via
+ * tpm_set_timing). */
+int tpm_get_stats(const tpm_ctx *ctx, tpm_stats *out);
+int tpm_set_timing(tpm_ctx *ctx, int enabled);
+
+/* Mesh::deposit (src/mesh.rs:62-69): cells (n_mesh^3) = CIC deposit of `mass` per particle,
+ * accumulated in 2^-32 fixed point (bit-exact run to run). */
+int tpm_deposit_f32(tpm_ctx *ctx, const float *pos, size_t n, float mass, float *cells);
+int tpm_deposit_f64(tpm_ctx *ctx, const double *pos, size_t n, double mass, float *cells);
+
+/* Simulation::density_contrast (src/integrator.rs:117-126). */
+int tpm_density_contrast_f32(tpm_ctx *ctx, const float *pos, size_t n, float *delta);
+
+/* PoissonSolver::acceleration (src/poisson.rs:33-88): delta (n_mesh^3) -> gx, gy, gz grids,
+ * filtered by exp(-k^2 r_split^2) with the given r_split (0 = unfiltered). */
+int tpm_pm_accel(tpm_ctx *ctx, const float *delta, double a, double r_split, float *gx, float *gy, float *gz);
+
+/* Mesh::interpolate (src/mesh.rs:72-77) of one grid at n positions. */
+int tpm_interpolate_f32(tpm_ctx *ctx, const float *cells, const float *pos, size_t n, float *out);
+
+/* tree::accelerations (src/tree.rs:314-323): short-range force only, coupling k given
+ * explicitly as in the reference. Uses the context's r_split, r_cut, softening, theta. */
+int tpm_short_range_f32(tpm_ctx *ctx, const float *pos, size_t n, double k, float *acc);
+int tpm_short_range_f64(tpm_ctx *ctx, const double *pos, size_t n, double k, float *acc);
+
+/*
+ * FP64 TreePM judge (no reference counterpart; the error yardstick for force accuracy):
+ * the full TreePM force in double precision (FP64 deposit, FFT, Green's function,
+ * interpolation) with the per-particle reference walk at opening angle `theta`, evaluated
+ * for the m particles listed in idx. acc: 3m doubles.
+ */
+int tpm_judge_f64(tpm_ctx *ctx, const double *pos, size_t n, const uint32_t *idx, size_t m, double a,
+                  double theta, double *acc);
+
+/*
+ * Zel'dovich initial conditions on the mesh_n^3 particle lattice (the structure of the
+ * reference's zeldovich, src/ics.rs:119-182): seeded counter-based white noise, power-law
+ * spectrum P(k) = k^-2 exp(-k/k_c), displacement rescaled to a 3D RMS of disp_rms,
+ * x = q + Psi wrapped into [0, L), p = mom_factor * Psi (mom_factor = a^2 f E,
+ * src/ics.rs:157-160). pos, mom: 3 mesh_n^3 doubles in lattice order (ix, iy, iz); mom may
+ * be NULL. Needs max_particles >= mesh_n^3.
+ */
+int tpm_zeldovich(tpm_ctx *ctx, uint64_t seed, double k_c, double disp_rms, double mom_factor, double *pos,
+                  double *mom);
+int tpm_zeldovich_dev(tpm_ctx *ctx, uint64_t seed, double k_c, double disp_rms, double mom_factor, double *d_pos,
+                      double *d_mom, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TPM_H */

@@ -1,0 +1,260 @@
+"""GPU parity suite: libtpm's HIP path against the CPU oracle, the golden vectors and the
+reference's own known-answer tests, at sizes the oracle finishes in seconds.
+
+Tolerances (north star: mass grid to 1e-6 relative, fixed-point grid bit-exact run to run;
+forces in FP32):
+  deposit   |rho - rho_ref| <= 1e-6 max(rho_ref, mean rho) per cell; bit-exact run to run and
+            under any permutation of the particles
+  PM force  within 2e-5 of the field's maximum vs the FP64 oracle (FP32 FFT round-off)
+  tree      theta = 0 within 1e-5 of max |a| of brute force (src/tree.rs:396-406)
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import OMEGA_M, lattice, max_diff, max_norm, mesh_scattered, rel_errors, tree_scattered
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def solver(treepm, n, box, split=None, max_particles=1 << 16, flags=0):
+    print(f"[gpu] context mesh={n} box={box} split={split is not None} flags={flags} cap={max_particles}", flush=True)
+    return treepm.Solver(n, box, OMEGA_M, split=split, max_particles=max_particles, flags=flags)
+
+
+# ----------------------------------------------------------------------------- deposit
+
+def test_deposit_kats(treepm):
+    s = solver(treepm, 8, 1.0)
+    h = 1.0 / 8
+    g = s.deposit(np.array([[2.5 * h, 3.5 * h, 4.5 * h]], np.float32)).reshape(8, 8, 8)
+    assert abs(g[2, 3, 4] - 1.0) < 1e-6 and (g > 1e-12).sum() == 1
+    g = s.deposit(lattice(8, 1.0), 3.0)
+    assert np.all(np.abs(g - 3.0) < 1e-6)
+    s4 = solver(treepm, 4, 1.0)
+    g = s4.deposit(np.array([[1.0, 0.125, 0.125]], np.float32)).reshape(4, 4, 4)
+    assert abs(g[3, 0, 0] - 0.5) < 1e-6 and abs(g[0, 0, 0] - 0.5) < 1e-6
+
+
+def test_deposit_matches_oracle_and_golden(treepm, oracle):
+    z = np.load(os.path.join(GOLD, "cic_5000.npz"))
+    s = solver(treepm, 16, 1.0)
+    g32 = s.deposit(z["pos"], 0.25)
+    g64 = s.deposit(z["pos"].astype(np.float64), 0.25)
+    ref = z["grid"]
+    scale = np.maximum(np.abs(ref), ref.mean())
+    assert np.max(np.abs(g64 - ref) / scale) < 1e-6
+    assert np.max(np.abs(g32 - ref) / scale) < 1e-6
+    o32 = oracle.deposit(z["pos"], 16, 1.0, 0.25)
+    assert np.max(np.abs(g32 - o32) / scale) < 1e-6
+    assert abs(g32.sum(dtype=np.float64) - 0.25 * 5000) < 1e-6 * 0.25 * 5000
+
+
+def test_deposit_bit_exact_and_order_independent(treepm):
+    rng = np.random.default_rng(1)
+    pos = (rng.random((50000, 3)) * 100.0).astype(np.float64)
+    pos[:5000] = 50.0 + rng.standard_normal((5000, 3))  # a dense clump
+    s = solver(treepm, 32, 100.0, max_particles=50000)
+    g1 = s.deposit(pos)
+    g2 = s.deposit(pos)
+    g3 = s.deposit(pos[rng.permutation(len(pos))])
+    assert np.array_equal(g1.view(np.uint32), g2.view(np.uint32))
+    assert np.array_equal(g1.view(np.uint32), g3.view(np.uint32))
+
+
+def test_density_contrast_matches_oracle(treepm, oracle):
+    pos = tree_scattered(20000, 100.0, 3)
+    s = solver(treepm, 32, 100.0, max_particles=20000)
+    d = s.density_contrast(pos)
+    ref = oracle.density_contrast(pos, 32, 100.0, dtype=np.float64)
+    assert np.max(np.abs(d - ref) / np.maximum(np.abs(ref + 1), 1.0)) < 1e-6
+
+
+# ------------------------------------------------------------------------- interpolate
+
+def test_interpolate_matches_oracle(treepm, oracle):
+    rng = np.random.default_rng(2)
+    cells = rng.standard_normal(16 ** 3).astype(np.float32)
+    pos = mesh_scattered(3000, 1.0)
+    s = solver(treepm, 16, 1.0)
+    v = s.interpolate(cells, pos)
+    ref = oracle.interpolate(cells, 16, 1.0, pos)
+    assert np.max(np.abs(v - ref)) <= 1e-6 * np.abs(cells).max()
+    v = s.interpolate(np.full(16 ** 3, 2.5, np.float32), pos)
+    assert np.all(np.abs(v - 2.5) < 1e-5)
+
+
+# ---------------------------------------------------------------------------- PM force
+
+def _mesh_from_fn(n, f):
+    x = (np.arange(n) + 0.5) / n
+    return np.broadcast_to(f(x)[:, None, None], (n, n, n)).astype(np.float32).ravel()
+
+
+def test_plane_wave_force_matches_analytic(treepm):
+    n, a = 16, 0.5
+    k0 = 2 * np.pi
+    c = 1.5 * OMEGA_M / a
+    s = solver(treepm, n, 1.0)
+    g = s.pm_accel(_mesh_from_fn(n, lambda x: np.cos(k0 * x)), a, 0.0)
+    exp = _mesh_from_fn(n, lambda x: -(c / k0) * np.sin(k0 * x))
+    amp = c / k0
+    assert np.max(np.abs(g[0] - exp)) < 1e-5 * amp
+    assert np.max(np.abs(g[1])) < 1e-5 * amp and np.max(np.abs(g[2])) < 1e-5 * amp
+
+
+def test_uniform_delta_and_inverse_a(treepm):
+    s = solver(treepm, 8, 1.0)
+    assert np.max(np.abs(s.pm_accel(np.ones(512, np.float32), 1.0))) < 1e-5
+    s16 = solver(treepm, 16, 1.0)
+    d = _mesh_from_fn(16, lambda x: np.cos(2 * np.pi * x))
+    r = np.abs(s16.pm_accel(d, 0.5)[0]).max() / np.abs(s16.pm_accel(d, 1.0)[0]).max()
+    assert abs(r - 2.0) < 1e-4
+
+
+@pytest.mark.parametrize("n", [4, 6, 8, 14, 16, 24, 30, 32, 47, 64])
+def test_pm_random_delta_matches_oracle(treepm, oracle, n):
+    rng = np.random.default_rng(n)
+    d = rng.standard_normal(n ** 3).astype(np.float32)
+    s = solver(treepm, n, 100.0)
+    rs = 1.25 * 100.0 / n
+    g = s.pm_accel(d, 0.3, rs)
+    ref = oracle.poisson_accel(d, n, 100.0, OMEGA_M, 0.3, rs, dtype=np.float64)
+    assert np.max(np.abs(g - ref)) < 2e-5 * np.abs(ref).max()
+
+
+# -------------------------------------------------------------------------- short range
+
+def test_tree_theta_zero_matches_brute_force(treepm):
+    z = np.load(os.path.join(GOLD, "bruteforce.npz"))
+    split = {"r_split": 0.5, "r_cut": 2.0, "softening": 0.05, "theta": 0.0}
+    for flags in (0, treepm.FLAG_REF_WALK):
+        s = solver(treepm, 8, 10.0, split, flags=flags)
+        fast = s.short_range(z["pos400"], 1.0)
+        assert max_diff(fast, z["acc400"]) / max_norm(z["acc400"]) < 1e-5
+
+
+def test_tree_theta_half_close_to_brute_force(treepm, oracle):
+    z = np.load(os.path.join(GOLD, "bruteforce.npz"))
+    split = {"r_split": 0.5, "r_cut": 2.5, "softening": 0.05, "theta": 0.5}
+    s = solver(treepm, 8, 10.0, split)
+    fast = s.short_range(z["pos3000"], 1.0)
+    assert max_diff(fast, z["acc3000"]) / max_norm(z["acc3000"]) < 0.02
+    # reference walk (same octree rule as the reference) against the oracle's tree
+    for flags in (treepm.FLAG_REF_WALK, treepm.FLAG_REF_WALK | treepm.FLAG_REF_LEAF_RULE):
+        s = solver(treepm, 8, 10.0, split, flags=flags)
+        ref_gpu = s.short_range(z["pos3000"], 1.0)
+        ora, _ = oracle.tree_accel(z["pos3000"], 10.0, 0.5, 2.5, 0.05, 0.5, 1.0)
+        assert max_diff(fast, z["acc3000"]) / max_norm(z["acc3000"]) < 0.02
+        if flags & treepm.FLAG_REF_LEAF_RULE:
+            assert max_diff(ref_gpu, ora) / max_norm(ora) < 1e-4
+
+
+def test_periodic_pair_interacts_across_boundary(treepm):
+    split = {"r_split": 0.5, "r_cut": 2.0, "softening": 0.05, "theta": 0.5}
+    s = solver(treepm, 8, 10.0, split)
+    acc = s.short_range(np.array([[0.1, 5.0, 5.0], [9.9, 5.0, 5.0]], np.float32), 1.0)
+    assert acc[0, 0] < 0 and acc[1, 0] > 0
+    assert abs(acc[0, 0] + acc[1, 0]) < 1e-6 * abs(acc[0, 0])
+
+
+# ---------------------------------------------------------------------- full TreePM
+
+def test_treepm_pair_force_matches_newton(treepm):
+    split = treepm.force_split(100.0, 64, 2, 0.05, 0.5)
+    split["softening"] = 0.01
+    s = solver(treepm, 64, 100.0, split)
+    acc, _ = s.accelerations(np.array([[50, 50, 50], [53, 50, 50]], np.float32), 1.0)
+    newton = treepm.coupling(OMEGA_M, 100.0, 1.0, 2) / 9.0
+    assert acc[0, 0] > 0 and acc[1, 0] < 0
+    assert abs(acc[0, 0] - newton) / newton < 0.05 and abs(acc[1, 0] + newton) / newton < 0.05
+
+
+def _zeldovich_config(treepm, n, seed=1):
+    s = solver(treepm, n, 100.0, max_particles=n ** 3)
+    pos, _ = s.zeldovich(seed, 2.0, 0.3 * 100.0 / n)
+    s.close()
+    return pos
+
+
+def test_zeldovich_matches_numpy_twin(treepm):
+    import ic_twin
+    n = 16
+    s = solver(treepm, n, 100.0, max_particles=n ** 3)
+    pos, mom = s.zeldovich(7, 2.0, 0.3 * 100.0 / n, mom_factor=0.5, want_momenta=True)
+    ref, rmom = ic_twin.zeldovich(n, 100.0, 7, 2.0, 0.3 * 100.0 / n, mom_factor=0.5)
+    d = pos - ref
+    d -= 100.0 * np.round(d / 100.0)
+    assert np.max(np.abs(d)) < 1e-9 * 100.0
+    assert np.max(np.abs(mom - rmom)) < 1e-9 * np.abs(rmom).max()
+    disp = mom / 0.5
+    assert abs(np.sqrt((disp ** 2).sum(1).mean()) - 0.3 * 100.0 / n) < 1e-9
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_full_treepm_matches_oracle(treepm, oracle, n):
+    pos = _zeldovich_config(treepm, n).astype(np.float32)
+    split = treepm.force_split(100.0, n, n, 1 / 30, 0.5)
+    a = 1.0 / 50.0
+    s = solver(treepm, n, 100.0, split, max_particles=n ** 3)
+    acc, delta = s.accelerations(pos, a, want_delta=True)
+    ref, dref, _ = oracle.accelerations(pos, n, 100.0, OMEGA_M, a, split, want_delta=True)
+    exact, _, _ = oracle.accelerations(pos.astype(np.float64), n, 100.0, OMEGA_M, a,
+                                       dict(split, theta=0.0), dtype=np.float64)
+    assert np.max(np.abs(delta - dref)) < 1e-5 * max(1.0, np.abs(dref).max())
+    e_gpu = rel_errors(acc, exact)
+    e_ora = rel_errors(ref, exact)
+    print(f"[gpu] n={n} rms rel err gpu={np.sqrt(np.mean(e_gpu**2)):.3e} oracle={np.sqrt(np.mean(e_ora**2)):.3e}")
+    # The group walk opens every node the reference opens, so it is at least as accurate.
+    assert np.sqrt(np.mean(e_gpu ** 2)) <= np.sqrt(np.mean(e_ora ** 2)) * 1.05 + 1e-5
+    assert np.sqrt(np.mean(rel_errors(acc, ref) ** 2)) < 0.01
+    # The reference-walk mode reproduces the oracle to FP32 round-off.
+    s2 = solver(treepm, n, 100.0, split, max_particles=n ** 3,
+                flags=treepm.FLAG_REF_WALK | treepm.FLAG_REF_LEAF_RULE)
+    acc2, _ = s2.accelerations(pos, a)
+    assert np.sqrt(np.mean(rel_errors(acc2, ref) ** 2)) < 1e-4
+
+
+def test_force_evaluation_is_bit_exact_run_to_run(treepm):
+    n = 32
+    pos = _zeldovich_config(treepm, n)
+    split = treepm.force_split(100.0, n, n, 1 / 30, 0.5)
+    s = solver(treepm, n, 100.0, split, max_particles=n ** 3)
+    a1, d1 = s.accelerations(pos, 0.02, want_delta=True)
+    a2, d2 = s.accelerations(pos, 0.02, want_delta=True)
+    assert np.array_equal(a1.view(np.uint32), a2.view(np.uint32))
+    assert np.array_equal(d1.view(np.uint32), d2.view(np.uint32))
+
+
+def test_judge_matches_fp64_oracle(treepm, oracle):
+    n = 16
+    pos = _zeldovich_config(treepm, n)
+    split = treepm.force_split(100.0, n, n, 1 / 30, 0.5)
+    a = 0.02
+    idx = np.arange(0, n ** 3, 7, dtype=np.uint32)
+    s = solver(treepm, n, 100.0, split, max_particles=n ** 3, flags=treepm.FLAG_REF_LEAF_RULE)
+    j = s.judge(pos, idx, a, theta=0.2)
+    ref, _, _ = oracle.accelerations(pos, n, 100.0, OMEGA_M, a, dict(split, theta=0.2), dtype=np.float64)
+    assert np.max(rel_errors(j, ref[idx])) < 1e-8
+
+
+def test_edge_cases(treepm):
+    split = treepm.force_split(100.0, 16, 16, 1 / 30, 0.5)
+    s = solver(treepm, 16, 100.0, split, max_particles=1000, flags=treepm.FLAG_CHECK_FINITE)
+    acc, _ = s.accelerations(np.zeros((0, 3), np.float32), 0.5)
+    assert acc.shape == (0, 3)
+    acc, _ = s.accelerations(np.array([[10.0, 20.0, 30.0]], np.float32), 0.5)
+    # One particle feels no force from its own density: CIC deposit and interpolation with an
+    # odd gradient kernel cancel exactly, up to FP32 round-off of the 4096x overdensity.
+    newton_one_cell = treepm.coupling(OMEGA_M, 100.0, 0.5, 1) / (100.0 / 16) ** 2
+    assert np.all(np.abs(acc) < 1e-4 * newton_one_cell)
+    # positions exactly on the box face and ragged counts
+    rng = np.random.default_rng(5)
+    pos = (rng.random((999, 3)) * 100).astype(np.float32)
+    pos[:3] = 100.0
+    acc, _ = s.accelerations(pos, 0.5)
+    assert np.all(np.isfinite(acc))
+    with pytest.raises(treepm.TpmError):
+        s.accelerations(np.zeros((1001, 3), np.float32), 0.5)
