@@ -9,6 +9,7 @@
 
 namespace tpm {
 size_t sort_temp_bytes(size_t max_n);
+size_t group_scan_bytes(size_t max_n);
 
 static std::string g_create_error;
 
@@ -54,7 +55,7 @@ static void free_all(Ctx &c) {
     void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.sort_tmp, c.spos,
                     c.counter, c.flag, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
                     c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.scan_tmp,
-                    c.scan_tmp_storage, c.inv, c.judge_buf};
+                    c.scan_tmp_storage, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -83,6 +84,10 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     c.sort_tmp_bytes = sort_temp_bytes(mn);
     TPM_HIP(hipMalloc(&c.sort_tmp, c.sort_tmp_bytes ? c.sort_tmp_bytes : 16));
     TPM_HIP(hipMalloc(&c.spos, 3 * mn * sizeof(double)));
+    TPM_HIP(hipMalloc(&c.gstart, (mn + 1) * sizeof(uint32_t)));
+    TPM_HIP(hipMalloc(&c.gscratch, 2 * mn * sizeof(uint32_t)));
+    c.gscan_bytes = group_scan_bytes(mn);
+    TPM_HIP(hipMalloc(&c.gscan_tmp, c.gscan_bytes ? c.gscan_bytes : 16));
     TPM_HIP(hipMalloc(&c.counter, 8 * sizeof(unsigned long long)));
     TPM_HIP(hipMalloc(&c.flag, sizeof(int) * 4));
     TPM_HIP(hipMalloc(&c.grid, c.m * sizeof(unsigned long long)));

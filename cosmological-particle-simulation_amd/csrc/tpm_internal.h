@@ -111,6 +111,13 @@ struct Ctx {
     size_t judge_cap = 0;
 
     Tree tree;
+    // Walk groups: particles [gstart[g], gstart[g+1]) of the Morton order.
+    uint32_t *gstart = nullptr;     // max_n + 1
+    uint32_t *gscratch = nullptr;   // 2 max_n words
+    void *gscan_tmp = nullptr;
+    size_t gscan_bytes = 0;
+    size_t ngroups = 0;
+    int group_level = 2;
     uint32_t *scan_tmp = nullptr;  // per-level child counts / offsets
     size_t scan_cap = 0;
     void *scan_tmp_storage = nullptr;
@@ -145,6 +152,11 @@ void interp_one(Ctx &c, const float *d_cells, const double *d_pos, size_t n, boo
 
 // Octree over the sorted particles (levels, ranges, boxes, centres of mass).
 void build_tree(Ctx &c, size_t n);
+
+// Walk groups: runs of at most 64 sorted particles that never cross a cube of octree level
+// c.group_level (sets c.gstart, c.ngroups).
+void build_groups(Ctx &c, size_t n);
+size_t group_scan_bytes(size_t max_n);
 
 // Short-range force of every particle added into acc[idx[i]] (group walk or reference walk).
 void walk_short_range(Ctx &c, size_t n, double k, float *d_acc);

@@ -9,6 +9,8 @@
 // with more than 8 particles finds its children's runs by binary search on the key digit of
 // the next level. Node cubes use the reference's FP32 centre/half arithmetic (centre +- half/2),
 // and centres of mass are accumulated bottom-up in FP64.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -275,6 +277,68 @@ void build_tree(Ctx &c, size_t n) {
         hipLaunchKernelGGL(k_com_level, dim3(blocks(e - b, 256)), dim3(256), 0, c.stream, c.spos, t.view(), b, e);
         TPM_HIP(hipGetLastError());
     }
+}
+
+// ------------------------------------------------------------------------- walk groups
+
+__global__ void k_group_seg(const uint64_t *__restrict__ keys, size_t n, int shift, uint32_t *__restrict__ segval) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool start = i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift);
+    segval[i] = start ? (uint32_t)i : 0u;
+}
+
+__global__ void k_group_chunk(const uint32_t *__restrict__ segstart, size_t n, uint32_t *__restrict__ flag) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = ((uint32_t)i - segstart[i]) % (uint32_t)kGroup == 0 ? 1u : 0u;
+}
+
+__global__ void k_group_starts(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ gid, size_t n,
+                               uint32_t *__restrict__ gstart) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (flag[i]) gstart[gid[i]] = (uint32_t)i;
+    if (i == n - 1) gstart[gid[i] + flag[i]] = (uint32_t)n;
+}
+
+size_t group_scan_bytes(size_t max_n) {
+    size_t a = 0, b = 0;
+    uint32_t *p = nullptr;
+    if (rocprim::inclusive_scan(nullptr, a, p, p, max_n, rocprim::maximum<uint32_t>()) != hipSuccess ||
+        rocprim::exclusive_scan(nullptr, b, p, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess)
+        throw Error{TPM_E_DEVICE, "scan size query failed"};
+    return std::max(a, b);
+}
+
+void build_groups(Ctx &c, size_t n) {
+    if (n == 0) {
+        c.ngroups = 0;
+        return;
+    }
+    // Group cubes hold about 512 particles at the mean density (8 particle spacings on a side
+    // when mesh = particle grid), and are never larger than L/4.
+    int level = 2;
+    while (level < kMaxDepth && ((double)n / std::pow(8.0, level + 1)) >= 512.0) ++level;
+    c.group_level = level;
+    const int shift = 3 * (kKeyBits - level);
+    uint32_t *segval = c.gscratch;
+    uint32_t *tmp = c.gscratch + c.max_n;
+    hipLaunchKernelGGL(k_group_seg, dim3(blocks(n, 256)), dim3(256), 0, c.stream, c.keys, n, shift, segval);
+    TPM_HIP(hipGetLastError());
+    size_t bytes = c.gscan_bytes;
+    TPM_HIP(rocprim::inclusive_scan(c.gscan_tmp, bytes, segval, tmp, n, rocprim::maximum<uint32_t>(), c.stream));
+    hipLaunchKernelGGL(k_group_chunk, dim3(blocks(n, 256)), dim3(256), 0, c.stream, tmp, n, segval);
+    TPM_HIP(hipGetLastError());
+    bytes = c.gscan_bytes;
+    TPM_HIP(rocprim::exclusive_scan(c.gscan_tmp, bytes, segval, tmp, 0u, n, rocprim::plus<uint32_t>(), c.stream));
+    hipLaunchKernelGGL(k_group_starts, dim3(blocks(n, 256)), dim3(256), 0, c.stream, segval, tmp, n, c.gstart);
+    TPM_HIP(hipGetLastError());
+    uint32_t last[2];
+    TPM_HIP(hipMemcpyAsync(&last[0], tmp + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    TPM_HIP(hipMemcpyAsync(&last[1], segval + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    TPM_HIP(hipStreamSynchronize(c.stream));
+    c.ngroups = (size_t)last[0] + last[1];
 }
 
 size_t sort_temp_bytes(size_t max_n) {

@@ -32,6 +32,7 @@ namespace tpm {
 struct WalkArgs {
     const double *spos;
     const uint32_t *idx;
+    const uint32_t *gstart;
     size_t n;
     TreeView t;
     double box;
@@ -327,9 +328,10 @@ __device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, size_t g0, do
 __global__ void __launch_bounds__(64) k_group_walk(WalkArgs w) {
     __shared__ WalkShared sh;
     const int lane = threadIdx.x;
-    const size_t g0 = (size_t)blockIdx.x * kGroup;
+    const size_t g0 = w.gstart[blockIdx.x];
+    const size_t g1 = w.gstart[blockIdx.x + 1];
     const size_t i = g0 + lane;
-    const bool active = i < w.n;
+    const bool active = i < g1;
     const size_t ii = active ? i : g0;
     const double px = w.spos[3 * ii], py = w.spos[3 * ii + 1], pz = w.spos[3 * ii + 2];
     const double mnx = wave_min(px), mny = wave_min(py), mnz = wave_min(pz);
@@ -500,7 +502,9 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc) {
     w.negk = (float)(-k);
     w.acc = d_acc;
     w.counter = c.counter;
-    hipLaunchKernelGGL(k_group_walk, dim3(blocks(n, kGroup)), dim3(kGroup), 0, c.stream, w);
+    build_groups(c, n);
+    w.gstart = c.gstart;
+    hipLaunchKernelGGL(k_group_walk, dim3((unsigned)c.ngroups), dim3(kGroup), 0, c.stream, w);
     TPM_HIP(hipGetLastError());
 }
 
