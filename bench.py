@@ -2,7 +2,7 @@
 
 Workload (default): 512^3 particles on a 512^3 mesh in a periodic 100 Mpc/h box, Zel'dovich
 ICs at z = 49 from a seeded Gaussian field with P(k) ~ k^-2 exp(-k / 2 h/Mpc) and a 0.3-cell
-3D displacement RMS, split r_s = 1.25 cells, r_cut = 4.5 r_s, theta = 0.5, softening L/(30*512).
+3D displacement RMS, split r_s = 1.25 cells, r_cut = 4.5 r_s, theta = 0.35, softening L/(30*512).
 A step is one full TreePM force evaluation (deposit -> r2c -> Green/gradient -> c2r -> CIC
 interpolation -> Morton sort/octree -> short-range walk) on positions resident in HBM.
 
@@ -35,7 +35,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_side=64, threads=None):
+def cpu_baseline(n_side=128, theta=0.35, threads=None):
     """Times the oracle (reference algorithm restated in C, OpenMP where the reference uses
     rayon, serial where it is serial) on one force evaluation of the 64^3 configuration."""
     import oracle
@@ -47,7 +47,7 @@ def cpu_baseline(n_side=64, threads=None):
     with treepm.Solver(n_side, BOX, OMEGA_M, max_particles=n_side ** 3) as ic:
         pos, _ = ic.zeldovich(1, K_C, 0.3 * BOX / n_side)
     pos = pos.astype(np.float32)
-    split = oracle.force_split(BOX, n_side, n_side, 1 / 30, 0.5)
+    split = oracle.force_split(BOX, n_side, n_side, 1 / 30, theta)
     a = 1.0 / (1.0 + Z_INIT)
     t0 = time.perf_counter()
     oracle.accelerations(pos, n_side, BOX, OMEGA_M, a, split)
@@ -64,11 +64,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=512, help="particles per side (= mesh cells per side)")
-    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--theta", type=float, default=0.35)
     ap.add_argument("--judge-sample", type=int, default=16384)
     ap.add_argument("--no-judge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-size", type=int, default=64)
+    ap.add_argument("--cpu-size", type=int, default=128)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -164,7 +164,7 @@ def main():
 
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
-        result["cpu_baseline"] = cpu_baseline(args.cpu_size)
+        result["cpu_baseline"] = cpu_baseline(args.cpu_size, theta=args.theta)
         log(f"[bench] CPU baseline {result['cpu_baseline']['value']:.3e} evals/s")
     print(json.dumps(result), flush=True)
 
