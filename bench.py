@@ -138,6 +138,8 @@ def main():
                    "softening": split["softening"], "positions": "f64", "pair_and_mesh_arithmetic": "f32"},
         "stages_ms": stages,
         "interactions_per_particle": inter / npart,
+        "walk_sources_per_group": {"groups": mean("groups"), "listed": mean("listed") / max(mean("groups"), 1),
+                                   "kept": mean("kept") / max(mean("groups"), 1)},
         "roofline": {"bound": "mfma", "kernel": "k_group_walk (FP32 VALU pair arithmetic)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},

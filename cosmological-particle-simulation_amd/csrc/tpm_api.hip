@@ -172,10 +172,13 @@ static void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w,
         c.stats.ms_tree = tm.ms(4, 5);
         c.stats.ms_walk = tm.ms(5, 6);
         c.stats.ms_total = tm.ms(0, 6);
-        unsigned long long cnt[2] = {0, 0};
+        unsigned long long cnt[4] = {0, 0, 0, 0};
         TPM_HIP(hipMemcpy(cnt, c.counter, sizeof(cnt), hipMemcpyDeviceToHost));
         c.stats.interactions = cnt[0];
         c.stats.nodes = c.tree.count;
+        c.stats.groups = c.ngroups;
+        c.stats.listed = cnt[2];
+        c.stats.kept = cnt[3];
         if (cnt[1]) throw Error{TPM_E_DEVICE, "walk stack overflow"};
     }
 }

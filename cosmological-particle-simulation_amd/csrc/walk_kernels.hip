@@ -36,9 +36,11 @@ struct WalkArgs {
     size_t n;
     TreeView t;
     double box;
+    double inv_box;
     double rcut;
     double theta2;
     float boxf;
+    float invboxf;
     float cut2;
     float eps2;
     float inv2rs;
@@ -56,21 +58,24 @@ constexpr float kErfA5 = 1.061405429f;
 constexpr float kTwoOverSqrtPi = 1.12837916709551257f;
 constexpr float kLog2e = 1.44269504088896341f;
 
-// S(r) for r = sqrt(r2), u = r * inv2rs.
+// S(r) for r = sqrt(r2), u = r * inv2rs. The hardware's single-instruction exp2 and
+// reciprocal (about 1 ulp) are used: exp2's argument stays above -40 for u < 5.
 __device__ __forceinline__ float short_range_factor_f32(float r, float inv2rs) {
     const float u = r * inv2rs;
-    const float e = exp2f(-(u * u) * kLog2e);
-    const float t = __frcp_rn(fmaf(kErfP, u, 1.0f));
+    const float e = __builtin_amdgcn_exp2f(-(u * u) * kLog2e);
+    const float t = __builtin_amdgcn_rcpf(fmaf(kErfP, u, 1.0f));
     const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, kErfA5, kErfA4), kErfA3), kErfA2), kErfA1);
     return e * fmaf(kTwoOverSqrtPi, u, poly);
 }
 
+// Pair force with single-instruction rsqrt/sqrt (about 1 ulp). s = r^2 + eps^2 > 0 whenever the
+// self pair is excluded or eps > 0.
 __device__ __forceinline__ void pair_f32(float dx, float dy, float dz, float r2, float mass, const WalkArgs &w,
                                          float &ax, float &ay, float &az) {
     const float s = r2 + w.eps2;
-    const float rs = __frsqrt_rn(s);
+    const float rs = __builtin_amdgcn_rsqf(s);
     const float inv3 = rs * rs * rs;
-    const float r = __fsqrt_rn(r2);
+    const float r = __builtin_amdgcn_sqrtf(r2);
     const float f = w.negk * mass * short_range_factor_f32(r, w.inv2rs) * inv3;
     ax = fmaf(dx, f, ax);
     ay = fmaf(dy, f, ay);
@@ -101,7 +106,7 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-__device__ __forceinline__ double min_image64(double d, double box) { return d - box * rint(d / box); }
+__device__ __forceinline__ double min_image64(double d, double box, double inv_box) { return d - box * rint(d * inv_box); }
 
 // Squared distance from point offset d (already minimum-imaged, relative to the box centre)
 // to an axis-aligned box of half extents h.
@@ -116,6 +121,7 @@ constexpr int kStackCap = 768;
 constexpr int kStackAlloc = kStackCap + 192;
 constexpr int kRangeCap = 128;
 constexpr int kNodeCap = 128;
+constexpr int kRing = 128;  // staging ring of kept sources (power of two, >= 127)
 
 struct WalkShared {
     uint32_t stack[kStackAlloc];
@@ -124,17 +130,54 @@ struct WalkShared {
     uint32_t nlist[kNodeCap];
     float4 tileA[64];
     float4 tileB[64];
+    float4 ring[kRing];
 };
 
-template <bool WRAP>
+// Pairs of every active lane with `cnt` staged sources at ring positions head, head+1, ...
+template <bool WRAP, bool SELF>
+__device__ __forceinline__ void eval_ring(const WalkShared &sh, int head, int cnt, const WalkArgs &w, uint32_t self,
+                                          float xi, float yi, float zi, float &ax, float &ay, float &az,
+                                          uint32_t &ninter) {
+#pragma unroll 2
+    for (int s = 0; s < cnt; ++s) {
+        const float4 src = sh.ring[(head + s) & (kRing - 1)];
+        float dx = xi - src.x;
+        float dy = yi - src.y;
+        float dz = zi - src.z;
+        if (WRAP) {
+            dx -= w.boxf * rintf(dx * w.invboxf);
+            dy -= w.boxf * rintf(dy * w.invboxf);
+            dz -= w.boxf * rintf(dz * w.invboxf);
+        }
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        bool take = r2 <= w.cut2;
+        if (SELF) take = take && __float_as_uint(src.w) != self;
+        if (take) {
+            pair_f32(dx, dy, dz, r2, 1.0f, w, ax, ay, az);
+            ++ninter;
+        }
+    }
+}
+
+// Direct sums over the particles of the listed leaf ranges. Each source is first tested
+// against the group's bounding box (half extents hx, hy, hz about the group centre): one
+// farther than r_cut from the whole box cannot be within r_cut of any particle of the group,
+// so it is dropped before the 64-lane evaluation. Kept sources are compacted into an LDS ring
+// and evaluated 64 at a time.
+template <bool WRAP, bool SELF>
 __device__ __forceinline__ void eval_particles(WalkShared &sh, int nr, int tot, const WalkArgs &w, double gx,
-                                               double gy, double gz, uint32_t self, float xi, float yi, float zi,
-                                               bool active, float &ax, float &ay, float &az, uint32_t &ninter) {
+                                               double gy, double gz, float hx, float hy, float hz, uint32_t self,
+                                               float xi, float yi, float zi, bool active, float &ax, float &ay,
+                                               float &az, uint32_t &ninter, uint32_t &kept) {
     const int lane = threadIdx.x;
     if (lane == 0) sh.r_off[nr] = (uint32_t)tot;
     __syncthreads();
+    const float cut2 = w.cut2 * (1.0f + 1e-5f);
+    int head = 0, cnt = 0;
     for (int base = 0; base < tot; base += 64) {
         const int e = base + lane;
+        bool keep = false;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (e < tot) {
             int lo = 0, hi = nr - 1;
             while (lo < hi) {
@@ -149,32 +192,38 @@ __device__ __forceinline__ void eval_particles(WalkShared &sh, int nr, int tot, 
             double dy = w.spos[3 * (size_t)j + 1] - gy;
             double dz = w.spos[3 * (size_t)j + 2] - gz;
             if (!WRAP) {
-                dx = min_image64(dx, w.box);
-                dy = min_image64(dy, w.box);
-                dz = min_image64(dz, w.box);
+                dx = min_image64(dx, w.box, w.inv_box);
+                dy = min_image64(dy, w.box, w.inv_box);
+                dz = min_image64(dz, w.box, w.inv_box);
             }
-            sh.tileA[lane] = make_float4((float)dx, (float)dy, (float)dz, __uint_as_float(j));
+            v = make_float4((float)dx, (float)dy, (float)dz, __uint_as_float(j));
+            if (WRAP) {
+                keep = true;
+            } else {
+                const float ex = fmaxf(fabsf(v.x) - hx, 0.f);
+                const float ey = fmaxf(fabsf(v.y) - hy, 0.f);
+                const float ez = fmaxf(fabsf(v.z) - hz, 0.f);
+                keep = fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= cut2;
+            }
         }
+        const unsigned long long m = __ballot(keep);
+        if (keep) {
+            const int at = __popcll(m & ((1ull << lane) - 1ull));
+            sh.ring[(head + cnt + at) & (kRing - 1)] = v;
+        }
+        cnt += __popcll(m);
         __syncthreads();
-        const int cnt = min(64, tot - base);
-        if (active) {
-            for (int s = 0; s < cnt; ++s) {
-                const float4 src = sh.tileA[s];
-                float dx = xi - src.x;
-                float dy = yi - src.y;
-                float dz = zi - src.z;
-                if (WRAP) {
-                    dx -= w.boxf * rintf(dx / w.boxf);
-                    dy -= w.boxf * rintf(dy / w.boxf);
-                    dz -= w.boxf * rintf(dz / w.boxf);
-                }
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                if (r2 <= w.cut2 && __float_as_uint(src.w) != self) {
-                    pair_f32(dx, dy, dz, r2, 1.0f, w, ax, ay, az);
-                    ++ninter;
-                }
-            }
+        if (cnt >= 64) {
+            if (active) eval_ring<WRAP, SELF>(sh, head, 64, w, self, xi, yi, zi, ax, ay, az, ninter);
+            head = (head + 64) & (kRing - 1);
+            cnt -= 64;
+            kept += 64;
+            __syncthreads();
         }
+    }
+    if (cnt > 0) {
+        if (active) eval_ring<WRAP, SELF>(sh, head, cnt, w, self, xi, yi, zi, ax, ay, az, ninter);
+        kept += cnt;
         __syncthreads();
     }
 }
@@ -194,12 +243,12 @@ __device__ __forceinline__ void eval_nodes(WalkShared &sh, int nn, const WalkArg
             double cx = cm.x - gx, cy = cm.y - gy, cz = cm.z - gz;
             double bxx = (double)bx.x - gx, byy = (double)bx.y - gy, bzz = (double)bx.z - gz;
             if (!WRAP) {
-                cx = min_image64(cx, w.box);
-                cy = min_image64(cy, w.box);
-                cz = min_image64(cz, w.box);
-                bxx = min_image64(bxx, w.box);
-                byy = min_image64(byy, w.box);
-                bzz = min_image64(bzz, w.box);
+                cx = min_image64(cx, w.box, w.inv_box);
+                cy = min_image64(cy, w.box, w.inv_box);
+                cz = min_image64(cz, w.box, w.inv_box);
+                bxx = min_image64(bxx, w.box, w.inv_box);
+                byy = min_image64(byy, w.box, w.inv_box);
+                bzz = min_image64(bzz, w.box, w.inv_box);
             }
             // src/tree.rs:275: max_r = r_cut + SQRT_3 * half, in FP32.
             const float max_r = (float)w.rcut + sqrt3 * bx.w;
@@ -213,17 +262,17 @@ __device__ __forceinline__ void eval_nodes(WalkShared &sh, int nn, const WalkArg
                 const float4 bb = sh.tileB[s];
                 float ux = bb.x - xi, uy = bb.y - yi, uz = bb.z - zi;
                 if (WRAP) {
-                    ux -= w.boxf * rintf(ux / w.boxf);
-                    uy -= w.boxf * rintf(uy / w.boxf);
-                    uz -= w.boxf * rintf(uz / w.boxf);
+                    ux -= w.boxf * rintf(ux * w.invboxf);
+                    uy -= w.boxf * rintf(uy * w.invboxf);
+                    uz -= w.boxf * rintf(uz * w.invboxf);
                 }
                 if (fmaf(ux, ux, fmaf(uy, uy, uz * uz)) > bb.w) continue;
                 const float4 src = sh.tileA[s];
                 float dx = xi - src.x, dy = yi - src.y, dz = zi - src.z;
                 if (WRAP) {
-                    dx -= w.boxf * rintf(dx / w.boxf);
-                    dy -= w.boxf * rintf(dy / w.boxf);
-                    dz -= w.boxf * rintf(dz / w.boxf);
+                    dx -= w.boxf * rintf(dx * w.invboxf);
+                    dy -= w.boxf * rintf(dy * w.invboxf);
+                    dz -= w.boxf * rintf(dz * w.invboxf);
                 }
                 const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                 pair_f32(dx, dy, dz, r2, src.w, w, ax, ay, az);
@@ -234,10 +283,12 @@ __device__ __forceinline__ void eval_nodes(WalkShared &sh, int nn, const WalkArg
     }
 }
 
-template <bool WRAP>
-__device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, size_t g0, double gx, double gy, double gz,
-                                double hx, double hy, double hz, uint32_t self, float xi, float yi, float zi,
-                                bool active, float &ax, float &ay, float &az, uint32_t &ninter) {
+template <bool WRAP, bool SELF>
+__device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, double gx, double gy, double gz, double hx,
+                                double hy, double hz, uint32_t self, float xi, float yi, float zi, bool active,
+                                float &ax, float &ay, float &az, uint32_t &ninter, uint32_t &listed,
+                                uint32_t &kept) {
+    const float hxf = (float)hx * (1.0f + 1e-6f), hyf = (float)hy * (1.0f + 1e-6f), hzf = (float)hz * (1.0f + 1e-6f);
     const int lane = threadIdx.x;
     const double sqrt3 = 1.7320508075688772;
     int size = 1;
@@ -261,9 +312,9 @@ __device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, size_t g0, do
             const float4 bx = w.t.box[node];
             double dx = (double)bx.x - gx, dy = (double)bx.y - gy, dz = (double)bx.z - gz;
             if (!WRAP) {
-                dx = min_image64(dx, w.box);
-                dy = min_image64(dy, w.box);
-                dz = min_image64(dz, w.box);
+                dx = min_image64(dx, w.box, w.inv_box);
+                dy = min_image64(dy, w.box, w.inv_box);
+                dz = min_image64(dz, w.box, w.inv_box);
             }
             const double maxr = w.rcut + sqrt3 * (double)bx.w;
             const double d2 = box_dist2(dx, dy, dz, hx, hy, hz);
@@ -274,9 +325,9 @@ __device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, size_t g0, do
                     const double4 cm = w.t.com[node];
                     double cx = cm.x - gx, cy = cm.y - gy, cz = cm.z - gz;
                     if (!WRAP) {
-                        cx = min_image64(cx, w.box);
-                        cy = min_image64(cy, w.box);
-                        cz = min_image64(cz, w.box);
+                        cx = min_image64(cx, w.box, w.inv_box);
+                        cy = min_image64(cy, w.box, w.inv_box);
+                        cz = min_image64(cz, w.box, w.inv_box);
                     }
                     const double dc2 = WRAP ? 0.0 : box_dist2(cx, cy, cz, hx, hy, hz);
                     const double side = 2.0 * (double)bx.w;
@@ -312,7 +363,9 @@ __device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, size_t g0, do
         nn += t_acc;
         __syncthreads();
         if (nr > kRangeCap - 64) {
-            eval_particles<WRAP>(sh, nr, tot, w, gx, gy, gz, self, xi, yi, zi, active, ax, ay, az, ninter);
+            eval_particles<WRAP, SELF>(sh, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active, ax, ay,
+                                       az, ninter, kept);
+            listed += (uint32_t)tot;
             nr = 0;
             tot = 0;
         }
@@ -321,7 +374,11 @@ __device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, size_t g0, do
             nn = 0;
         }
     }
-    if (nr > 0) eval_particles<WRAP>(sh, nr, tot, w, gx, gy, gz, self, xi, yi, zi, active, ax, ay, az, ninter);
+    if (nr > 0) {
+        eval_particles<WRAP, SELF>(sh, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active, ax, ay, az,
+                                   ninter, kept);
+        listed += (uint32_t)tot;
+    }
     if (nn > 0) eval_nodes<WRAP>(sh, nn, w, gx, gy, gz, xi, yi, zi, active, ax, ay, az, ninter);
 }
 
@@ -348,11 +405,25 @@ __global__ void __launch_bounds__(64) k_group_walk(WalkArgs w) {
     // node distance is wrapped individually.
     const double lim = 0.5 * w.box * (1.0 - 1e-9);
     const bool wrap = (hx + w.rcut >= lim) || (hy + w.rcut >= lim) || (hz + w.rcut >= lim);
-    if (wrap)
-        group_walk_body<true>(sh, w, g0, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay, az, ninter);
-    else
-        group_walk_body<false>(sh, w, g0, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay, az,
-                               ninter);
+    // With softening > 0 a particle's pair with itself has d = 0 and a finite kernel, so it adds
+    // exactly zero (as the reference's j == i skip does); the index test is needed only at eps = 0.
+    uint32_t listed = 0, kept = 0;
+    const bool self_test = !(w.eps2 > 0.0f);
+    if (wrap) {
+        if (self_test)
+            group_walk_body<true, true>(sh, w, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay, az,
+                                        ninter, listed, kept);
+        else
+            group_walk_body<true, false>(sh, w, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay, az,
+                                         ninter, listed, kept);
+    } else {
+        if (self_test)
+            group_walk_body<false, true>(sh, w, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay, az,
+                                         ninter, listed, kept);
+        else
+            group_walk_body<false, false>(sh, w, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay,
+                                          az, ninter, listed, kept);
+    }
     if (active) {
         const size_t o = w.idx[i];
         w.acc[3 * o] += ax;
@@ -361,7 +432,11 @@ __global__ void __launch_bounds__(64) k_group_walk(WalkArgs w) {
     }
     int tot;
     wave_excl_scan(active ? (int)ninter : 0, tot);
-    if (lane == 0 && w.counter) atomicAdd(w.counter, (unsigned long long)tot);
+    if (lane == 0 && w.counter) {
+        atomicAdd(w.counter, (unsigned long long)tot);
+        atomicAdd(w.counter + 2, (unsigned long long)listed);
+        atomicAdd(w.counter + 3, (unsigned long long)kept);
+    }
 }
 
 // -------------------------------------------------------------------- reference walk
@@ -493,9 +568,11 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc) {
     w.n = n;
     w.t = c.tree.view();
     w.box = c.p.box_size;
+    w.inv_box = 1.0 / c.p.box_size;
     w.rcut = c.p.r_cut;
     w.theta2 = c.p.theta * c.p.theta;
     w.boxf = (float)c.p.box_size;
+    w.invboxf = (float)(1.0 / c.p.box_size);
     w.cut2 = (float)c.p.r_cut * (float)c.p.r_cut;
     w.eps2 = (float)c.p.softening * (float)c.p.softening;
     w.inv2rs = (float)(1.0 / (2.0 * c.p.r_split));

@@ -75,6 +75,9 @@ class Stats(ctypes.Structure):
         ("ms_walk", ctypes.c_double),
         ("interactions", ctypes.c_uint64),
         ("nodes", ctypes.c_uint64),
+        ("groups", ctypes.c_uint64),
+        ("listed", ctypes.c_uint64),
+        ("kept", ctypes.c_uint64),
     ]
 
     def as_dict(self):

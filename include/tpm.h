@@ -72,6 +72,9 @@ typedef struct tpm_stats {
     double ms_walk;        /* short-range walk */
     uint64_t interactions; /* particle-particle + particle-node interactions evaluated */
     uint64_t nodes;        /* octree nodes */
+    uint64_t groups;       /* walk groups (wavefronts) */
+    uint64_t listed;       /* particle sources listed by the group walks (before the box test) */
+    uint64_t kept;         /* particle sources kept for evaluation (each costs one 64-lane step) */
 } tpm_stats;
 
 /* Version string of the library. */
