@@ -75,38 +75,35 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
-    import torch
-
     import treepm
 
     n = args.size
     npart = n ** 3
     a = 1.0 / (1.0 + Z_INIT)
     split = treepm.force_split(BOX, n, n, 1 / 30, args.theta)
-    dev = torch.device("cuda:0")
     log(f"[bench] {npart} particles, mesh {n}^3, theta {args.theta}")
 
     solver = treepm.Solver(n, BOX, OMEGA_M, split=split, max_particles=npart)
-    pos = torch.empty(3 * npart, dtype=torch.float64, device=dev)
-    acc = torch.empty(3 * npart, dtype=torch.float32, device=dev)
-    solver.zeldovich_dev(1, K_C, 0.3 * BOX / n, 0.0, pos.data_ptr())
-    torch.cuda.synchronize()
+    pos = treepm.DeviceArray(3 * npart, np.float64)
+    acc = treepm.DeviceArray(3 * npart, np.float32)
+    solver.zeldovich_dev(1, K_C, 0.3 * BOX / n, 0.0, pos.ptr)
+    solver.sync()
     log("[bench] ICs generated")
 
     for i in range(args.warmup):
-        solver.accelerations_dev(pos.data_ptr(), npart, a, acc.data_ptr())
-        torch.cuda.synchronize()
+        solver.accelerations_dev(pos.ptr, npart, a, acc.ptr)
+        solver.sync()
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
 
     solver.set_timing(True)
     stats = []
-    torch.cuda.synchronize()
+    solver.sync()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        solver.accelerations_dev(pos.data_ptr(), npart, a, acc.data_ptr())
+        solver.accelerations_dev(pos.ptr, npart, a, acc.ptr)
         stats.append(solver.stats())
         log(f"[bench] step {i + 1}/{args.steps}: {stats[-1]['ms_total']:.1f} ms")
-    torch.cuda.synchronize()
+    solver.sync()
     wall = time.perf_counter() - t0
     ms_per_step = 1000.0 * wall / args.steps
     solver.set_timing(False)
@@ -149,8 +146,8 @@ def main():
         m = min(args.judge_sample, npart)
         rng = np.random.default_rng(12345)
         idx = np.sort(rng.choice(npart, size=m, replace=False)).astype(np.uint32)
-        host_pos = pos.cpu().numpy().reshape(-1, 3)
-        host_acc = acc.cpu().numpy().reshape(-1, 3)
+        host_pos = pos.numpy().reshape(-1, 3)
+        host_acc = acc.numpy().reshape(-1, 3)
         log("[bench] FP64 judge ...")
         ref = solver.judge(host_pos, idx, a, theta=0.2)
         d = host_acc[idx].astype(np.float64) - ref
@@ -161,8 +158,8 @@ def main():
         log(f"[bench] force error rms {result['force_error']['rms']:.3e} p99 {result['force_error']['p99']:.3e}")
 
     solver.close()
-    del pos, acc
-    torch.cuda.empty_cache()
+    pos.free()
+    acc.free()
 
     if not args.no_cpu:
         log("[bench] CPU baseline ...")

@@ -8,6 +8,7 @@
 // the remainder so each particle adds exactly round(mass * 2^32), and the quanta are summed
 // with 64-bit integer atomics. Integer addition commutes, so the grid is bit-exact from run to
 // run whatever the order of the atomics, and the total mass is exact.
+#include <climits>
 #include <cmath>
 
 #include "tpm_internal.h"
@@ -76,6 +77,107 @@ __global__ void __launch_bounds__(256) k_deposit(const double *__restrict__ pos,
     // Remainder to the last corner; modular arithmetic keeps the cell sums exact even if a
     // rounding makes this quantum "negative".
     atomicAdd(grid + st.idx[7], total - used);
+}
+
+// Same quanta as k_deposit, accumulated per workgroup in an LDS tile first. A workgroup takes
+// kDepChunk consecutive particles of the Morton order, which cover a compact block of cells; if
+// that block (plus the +1 CIC corner) fits the tile, the 8 quanta of each particle are added
+// with LDS atomics and each non-zero tile cell is then added to the grid with one global atomic.
+// Chunks whose block does not fit (they straddle a large Morton boundary) add directly. Integer
+// sums commute, so the grid is bit-identical to k_deposit's.
+constexpr int kDepChunk = 1024;
+constexpr int kDepTile = 4096;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_deposit_tiled(const double *__restrict__ pos, size_t np, int n, double box,
+                                                       double mass, unsigned long long *__restrict__ grid) {
+    __shared__ unsigned long long tile[kDepTile];
+    __shared__ int red[6][8];
+    const size_t c0 = (size_t)blockIdx.x * kDepChunk;
+    const int tid = threadIdx.x;
+    const T h = (T)box / (T)n;
+    // Pass 1: bounding block of the stencil bases (unwrapped cell indices, base in [-1, n-1]).
+    int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (int k = tid; k < kDepChunk; k += 256) {
+        const size_t i = c0 + k;
+        if (i >= np) break;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const T sv = (T)pos[3 * i + a] / h - (T)0.5;
+            const int b = (int)floor(sv);
+            mn[a] = min(mn[a], b);
+            mx[a] = max(mx[a], b);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        for (int d = 32; d >= 1; d >>= 1) {
+            mn[a] = min(mn[a], __shfl_xor(mn[a], d, 64));
+            mx[a] = max(mx[a], __shfl_xor(mx[a], d, 64));
+        }
+    }
+    const int wave = tid >> 6, lane = tid & 63;
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) {
+            red[a][wave] = mn[a];
+            red[3 + a][wave] = mx[a];
+        }
+    __syncthreads();
+    int lo[3], dim[3];
+    for (int a = 0; a < 3; ++a) {
+        int m0 = red[a][0], m1 = red[3 + a][0];
+        for (int wv = 1; wv < 4; ++wv) {
+            m0 = min(m0, red[a][wv]);
+            m1 = max(m1, red[3 + a][wv]);
+        }
+        lo[a] = m0;
+        dim[a] = m1 - m0 + 2;
+    }
+    const long long cells = (long long)dim[0] * dim[1] * dim[2];
+    const double scale = mass * 0x1p32;
+    const unsigned long long total = (unsigned long long)llrint(scale);
+    const bool use_tile = cells <= kDepTile && dim[0] > 0;
+    if (use_tile) {
+        for (int c = tid; c < (int)cells; c += 256) tile[c] = 0ull;
+        __syncthreads();
+    }
+    for (int k = tid; k < kDepChunk; k += 256) {
+        const size_t i = c0 + k;
+        if (i >= np) break;
+        Stencil<T> st;
+        cic_stencil<T>((T)pos[3 * i], (T)pos[3 * i + 1], (T)pos[3 * i + 2], n, (T)box, st);
+        int b[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) b[a] = (int)floor((T)pos[3 * i + a] / h - (T)0.5) - lo[a];
+        unsigned long long used = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const unsigned long long q =
+                c < 7 ? (unsigned long long)llrint((double)st.w[c] * scale) : total - used;
+            used += q;
+            if (use_tile) {
+                const int dx = (c >> 2) & 1, dy = (c >> 1) & 1, dz = c & 1;
+                const int li = ((b[0] + dx) * dim[1] + (b[1] + dy)) * dim[2] + (b[2] + dz);
+                atomicAdd(&tile[li], q);
+            } else {
+                atomicAdd(grid + st.idx[c], q);
+            }
+        }
+    }
+    if (!use_tile) return;
+    __syncthreads();
+    for (int c = tid; c < (int)cells; c += 256) {
+        const unsigned long long v = tile[c];
+        if (!v) continue;
+        const int iz = c % dim[2];
+        const int iy = (c / dim[2]) % dim[1];
+        const int ix = c / (dim[2] * dim[1]);
+        int gx = (lo[0] + ix) % n, gy = (lo[1] + iy) % n, gz = (lo[2] + iz) % n;
+        gx += gx < 0 ? n : 0;
+        gy += gy < 0 ? n : 0;
+        gz += gz < 0 ? n : 0;
+        atomicAdd(grid + ((size_t)gx * n + gy) * n + gz, v);
+    }
 }
 
 __global__ void k_grid_to_float(const unsigned long long *__restrict__ grid, size_t m, float mean32,
@@ -151,9 +253,20 @@ __global__ void k_check_finite(const float *__restrict__ v, size_t count, int *f
 
 static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double *pos_src) {
+void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double *pos_src, bool sorted) {
     TPM_HIP(hipMemsetAsync(c.grid, 0, c.m * sizeof(unsigned long long), c.stream));
     if (n == 0) return;
+    if (sorted) {
+        const unsigned nb = (unsigned)((n + kDepChunk - 1) / kDepChunk);
+        if (f32_weights)
+            hipLaunchKernelGGL(k_deposit_tiled<float>, dim3(nb), dim3(256), 0, c.stream, pos_src, n, c.n,
+                               c.p.box_size, mass, c.grid);
+        else
+            hipLaunchKernelGGL(k_deposit_tiled<double>, dim3(nb), dim3(256), 0, c.stream, pos_src, n, c.n,
+                               c.p.box_size, mass, c.grid);
+        TPM_HIP(hipGetLastError());
+        return;
+    }
     if (f32_weights)
         hipLaunchKernelGGL(k_deposit<float>, dim3(blocks(n, 256)), dim3(256), 0, c.stream, pos_src, n, c.n,
                            c.p.box_size, mass, c.grid);

@@ -143,7 +143,7 @@ static void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w,
     tm.mark();
     sort_particles(c, d_pos, n);
     tm.mark();
-    deposit_fixed(c, n, 1.0, f32w, c.spos);
+    deposit_fixed(c, n, 1.0, f32w, c.spos, true);
     tm.mark();
     if (n > 0) {
         pm_from_grid(c, n, a, treepm ? c.p.r_split : 0.0, d_delta);
@@ -335,7 +335,7 @@ int tpm_deposit_f32(tpm_ctx *ctx, const float *pos, size_t n, float mass, float 
     return guarded(ctx, [&](Ctx &c) {
         TPM_REQUIRE(cells != nullptr, "cells pointer is null");
         tpm::upload_positions(c, pos, n, true);
-        tpm::deposit_fixed(c, n, (double)mass, true, c.pos);
+        tpm::deposit_fixed(c, n, (double)mass, true, c.pos, false);
         tpm::grid_to_float(c, n ? n : 1, nullptr, c.delta);
         TPM_HIP(hipMemcpyAsync(cells, c.delta, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
         TPM_HIP(hipStreamSynchronize(c.stream));
@@ -347,7 +347,7 @@ int tpm_deposit_f64(tpm_ctx *ctx, const double *pos, size_t n, double mass, floa
     return guarded(ctx, [&](Ctx &c) {
         TPM_REQUIRE(cells != nullptr, "cells pointer is null");
         tpm::upload_positions(c, pos, n, false);
-        tpm::deposit_fixed(c, n, mass, false, c.pos);
+        tpm::deposit_fixed(c, n, mass, false, c.pos, false);
         tpm::grid_to_float(c, n ? n : 1, nullptr, c.delta);
         TPM_HIP(hipMemcpyAsync(cells, c.delta, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
         TPM_HIP(hipStreamSynchronize(c.stream));
@@ -360,7 +360,7 @@ int tpm_density_contrast_f32(tpm_ctx *ctx, const float *pos, size_t n, float *de
         TPM_REQUIRE(delta != nullptr, "delta pointer is null");
         TPM_REQUIRE(n > 0, "density contrast needs at least one particle");
         tpm::upload_positions(c, pos, n, true);
-        tpm::deposit_fixed(c, n, 1.0, true, c.pos);
+        tpm::deposit_fixed(c, n, 1.0, true, c.pos, false);
         tpm::grid_to_float(c, n, c.delta, nullptr);
         TPM_HIP(hipMemcpyAsync(delta, c.delta, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
         TPM_HIP(hipStreamSynchronize(c.stream));
@@ -432,7 +432,7 @@ int tpm_judge_f64(tpm_ctx *ctx, const double *pos, size_t n, const uint32_t *idx
         TPM_REQUIRE(theta >= 0 && std::isfinite(theta), "theta must be >= 0");
         tpm::upload_positions(c, pos, n, false);
         tpm::sort_particles(c, c.pos, n);
-        tpm::deposit_fixed(c, n, 1.0, false, c.spos);
+        tpm::deposit_fixed(c, n, 1.0, false, c.spos, true);
         const bool treepm = (c.p.flags & TPM_FLAG_TREEPM) != 0;
         tpm::pm_from_grid_f64(c, n, a, treepm ? c.p.r_split : 0.0);
         if (!c.inv) TPM_HIP(hipMalloc(&c.inv, (c.max_n ? c.max_n : 1) * sizeof(uint32_t)));
@@ -464,6 +464,104 @@ int tpm_judge_f64(tpm_ctx *ctx, const double *pos, size_t n, const uint32_t *idx
         TPM_HIP(hipMemcpyAsync(acc, d_pm, 3 * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         TPM_HIP(hipStreamSynchronize(c.stream));
         TPM_REQUIRE(!bad, "judge target index out of range");
+        return TPM_OK;
+    });
+}
+
+int tpm_device_alloc(size_t bytes, void **out) {
+    if (!out) return TPM_E_INVALID;
+    *out = nullptr;
+    if (bytes == 0) return TPM_OK;
+    return hipMalloc(out, bytes) == hipSuccess ? TPM_OK : TPM_E_DEVICE;
+}
+
+int tpm_device_free(void *ptr) {
+    if (!ptr) return TPM_OK;
+    return hipFree(ptr) == hipSuccess ? TPM_OK : TPM_E_DEVICE;
+}
+
+int tpm_memcpy(void *dst, const void *src, size_t bytes, int kind) {
+    if (bytes == 0) return TPM_OK;
+    if (!dst || !src || kind < 0 || kind > 2) return TPM_E_INVALID;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                          : hipMemcpyDeviceToDevice;
+    return hipMemcpy(dst, src, bytes, k) == hipSuccess ? TPM_OK : TPM_E_DEVICE;
+}
+
+int tpm_sync(tpm_ctx *ctx) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        return TPM_OK;
+    });
+}
+
+int tpm_kdk_factors(const tpm_cosmology *cosmo, double a0, double a1, double *am, double *k1, double *d, double *k2) {
+    if (!cosmo || !(a0 > 0) || !(a1 > 0) || !std::isfinite(a0) || !std::isfinite(a1)) return TPM_E_INVALID;
+    const tpm::KdkFactors f = tpm::kdk_factors(*cosmo, a0, a1);
+    if (am) *am = f.am;
+    if (k1) *k1 = f.k1;
+    if (d) *d = f.d;
+    if (k2) *k2 = f.k2;
+    return TPM_OK;
+}
+
+int tpm_step_to_dev(tpm_ctx *ctx, const tpm_cosmology *cosmo, double *d_pos, double *d_mom, size_t n, double a0,
+                    double a1, float *d_acc, float *d_delta, void *stream) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(cosmo != nullptr, "null cosmology");
+        TPM_REQUIRE(n == 0 || (d_pos && d_mom && d_acc), "null device pointer");
+        TPM_REQUIRE(a0 > 0 && a1 > 0 && std::isfinite(a0) && std::isfinite(a1), "scale factors must be positive");
+        hipStream_t saved = c.stream;
+        if (stream) c.stream = (hipStream_t)stream;
+        try {
+            const tpm::KdkFactors f = tpm::kdk_factors(*cosmo, a0, a1);
+            tpm::kick(c, d_mom, d_acc, n, f.k1);
+            tpm::drift(c, d_pos, d_mom, n, f.d);
+            tpm::evaluate(c, d_pos, n, a1, false, d_acc, d_delta);
+            tpm::kick(c, d_mom, d_acc, n, f.k2);
+        } catch (...) {
+            c.stream = saved;
+            throw;
+        }
+        c.stream = saved;
+        return TPM_OK;
+    });
+}
+
+int tpm_run(tpm_ctx *ctx, const tpm_cosmology *cosmo, double *pos, double *mom, size_t n, double a_start,
+            double a_end, int steps) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(cosmo != nullptr, "null cosmology");
+        TPM_REQUIRE(n == 0 || (pos && mom), "null host pointer");
+        TPM_REQUIRE(steps >= 0, "steps must be >= 0");
+        TPM_REQUIRE(a_start > 0 && a_end > 0, "scale factors must be positive");
+        TPM_REQUIRE(n <= c.max_n, "more particles than the context capacity");
+        if (n == 0) return TPM_OK;
+        double *d_mom = nullptr;
+        TPM_HIP(hipMalloc(&d_mom, 3 * n * sizeof(double)));
+        try {
+            TPM_HIP(hipMemcpyAsync(c.pos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+            TPM_HIP(hipMemcpyAsync(d_mom, mom, 3 * n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+            tpm::evaluate(c, c.pos, n, a_start, false, c.acc, nullptr);
+            double a = a_start;
+            for (int i = 0; i < steps; ++i) {
+                const double a1 = a_start * std::pow(a_end / a_start, (double)(i + 1) / (double)steps);
+                const tpm::KdkFactors f = tpm::kdk_factors(*cosmo, a, a1);
+                tpm::kick(c, d_mom, c.acc, n, f.k1);
+                tpm::drift(c, c.pos, d_mom, n, f.d);
+                tpm::evaluate(c, c.pos, n, a1, false, c.acc, nullptr);
+                tpm::kick(c, d_mom, c.acc, n, f.k2);
+                a = a1;
+            }
+            tpm::finish_checks(c);
+            TPM_HIP(hipMemcpyAsync(pos, c.pos, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            TPM_HIP(hipMemcpyAsync(mom, d_mom, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            TPM_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            (void)hipFree(d_mom);
+            throw;
+        }
+        (void)hipFree(d_mom);
         return TPM_OK;
     });
 }

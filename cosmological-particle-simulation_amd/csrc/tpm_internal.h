@@ -131,9 +131,10 @@ struct Ctx {
 // then spos = positions gathered into key order.
 void sort_particles(Ctx &c, const double *d_pos, size_t n);
 
-// Fixed-point CIC deposit of sorted positions into c.grid. f32_weights: compute the stencil
-// in FP32 from the positions rounded to float (the reference's arithmetic).
-void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double *pos_src);
+// Fixed-point CIC deposit of positions into c.grid. f32_weights: compute the stencil in FP32
+// from the positions rounded to float (the reference's arithmetic). sorted: positions are in
+// Morton order (enables the LDS-tiled kernel; the grid is bit-identical either way).
+void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double *pos_src, bool sorted);
 // grid -> density contrast (FP32, the reference's c/mean - 1) and/or mass grid.
 void grid_to_float(Ctx &c, size_t n_particles, float *delta_out, float *mass_out);
 
@@ -177,6 +178,14 @@ void displacement_from_noise(Ctx &c, const float *d_noise, double kc);
 // Zel'dovich lattice ICs (ic_kernels.hip): positions and momenta (3 n^3 doubles each, lattice
 // order) on the device.
 void zeldovich(Ctx &c, uint64_t seed, double kc, double disp_rms, double mom_factor, double *d_pos, double *d_mom);
+
+// KDK (kdk_kernels.hip): step factors in FP64 and the device kick / drift.
+struct KdkFactors {
+    double am, k1, d, k2;
+};
+KdkFactors kdk_factors(const tpm_cosmology &c, double a0, double a1);
+void kick(Ctx &c, double *d_mom, const float *d_acc, size_t n, double f);
+void drift(Ctx &c, double *d_pos, const double *d_mom, size_t n, double f);
 
 // Plans and tables (fft_kernels.hip).
 void make_plan(Ctx &c);

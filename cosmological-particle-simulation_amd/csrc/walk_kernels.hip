@@ -134,10 +134,13 @@ struct WalkShared {
 };
 
 // Pairs of every active lane with `cnt` staged sources at ring positions head, head+1, ...
+// (Sources are broadcast from LDS; a lane-private compaction of the accepted pairs does not pay
+// here because consecutive sources are spatially coherent, so some lane accepts most of a tile.)
 template <bool WRAP, bool SELF>
 __device__ __forceinline__ void eval_ring(const WalkShared &sh, int head, int cnt, const WalkArgs &w, uint32_t self,
-                                          float xi, float yi, float zi, float &ax, float &ay, float &az,
-                                          uint32_t &ninter) {
+                                          float xi, float yi, float zi, bool active, float &ax, float &ay,
+                                          float &az, uint32_t &ninter) {
+    if (!active) return;
 #pragma unroll 2
     for (int s = 0; s < cnt; ++s) {
         const float4 src = sh.ring[(head + s) & (kRing - 1)];
@@ -214,7 +217,7 @@ __device__ __forceinline__ void eval_particles(WalkShared &sh, int nr, int tot, 
         cnt += __popcll(m);
         __syncthreads();
         if (cnt >= 64) {
-            if (active) eval_ring<WRAP, SELF>(sh, head, 64, w, self, xi, yi, zi, ax, ay, az, ninter);
+            eval_ring<WRAP, SELF>(sh, head, 64, w, self, xi, yi, zi, active, ax, ay, az, ninter);
             head = (head + 64) & (kRing - 1);
             cnt -= 64;
             kept += 64;
@@ -222,7 +225,7 @@ __device__ __forceinline__ void eval_particles(WalkShared &sh, int nr, int tot, 
         }
     }
     if (cnt > 0) {
-        if (active) eval_ring<WRAP, SELF>(sh, head, cnt, w, self, xi, yi, zi, ax, ay, az, ninter);
+        eval_ring<WRAP, SELF>(sh, head, cnt, w, self, xi, yi, zi, active, ax, ay, az, ninter);
         kept += cnt;
         __syncthreads();
     }

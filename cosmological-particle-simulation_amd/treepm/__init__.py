@@ -41,7 +41,8 @@ EXPORTS = [
     "tpm_get_stats", "tpm_set_timing", "tpm_deposit_f32", "tpm_deposit_f64",
     "tpm_density_contrast_f32", "tpm_pm_accel", "tpm_interpolate_f32",
     "tpm_short_range_f32", "tpm_short_range_f64", "tpm_judge_f64",
-    "tpm_zeldovich", "tpm_zeldovich_dev",
+    "tpm_zeldovich", "tpm_zeldovich_dev", "tpm_kdk_factors", "tpm_step_to_dev", "tpm_run",
+    "tpm_device_alloc", "tpm_device_free", "tpm_memcpy", "tpm_sync",
 ]
 
 
@@ -49,6 +50,20 @@ class TpmError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"tpm error {code}: {msg}")
         self.code = code
+
+
+class Cosmology(ctypes.Structure):
+    _fields_ = [("omega_r", ctypes.c_double), ("omega_m", ctypes.c_double), ("omega_l", ctypes.c_double)]
+
+    @classmethod
+    def lcdm_default(cls):
+        """Cosmology::lcdm_default (src/cosmology.rs:21-30), the float32 values of the reference."""
+        f = np.float32
+        c = cls()
+        c.omega_r = float(f(9.14e-5))
+        c.omega_m = float(f(0.315))
+        c.omega_l = float(f(1.0) - f(0.315) - f(9.14e-5))
+        return c
 
 
 class Params(ctypes.Structure):
@@ -117,6 +132,13 @@ def load():
         "tpm_judge_f64": (ctypes.c_int, [P, P, S, P, S, D, D, P]),
         "tpm_zeldovich": (ctypes.c_int, [P, ctypes.c_uint64, D, D, D, P, P]),
         "tpm_zeldovich_dev": (ctypes.c_int, [P, ctypes.c_uint64, D, D, D, P, P, P]),
+        "tpm_kdk_factors": (ctypes.c_int, [ctypes.POINTER(Cosmology), D, D, P, P, P, P]),
+        "tpm_step_to_dev": (ctypes.c_int, [P, ctypes.POINTER(Cosmology), P, P, S, D, D, P, P, P]),
+        "tpm_run": (ctypes.c_int, [P, ctypes.POINTER(Cosmology), P, P, S, D, D, ctypes.c_int]),
+        "tpm_device_alloc": (ctypes.c_int, [S, ctypes.POINTER(ctypes.c_void_p)]),
+        "tpm_device_free": (ctypes.c_int, [P]),
+        "tpm_memcpy": (ctypes.c_int, [P, P, S, ctypes.c_int]),
+        "tpm_sync": (ctypes.c_int, [P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -288,6 +310,25 @@ class Solver:
                                                ctypes.c_void_p(d_pos), ctypes.c_void_p(d_mom or None),
                                                ctypes.c_void_p(stream or None)))
 
+    def run(self, pos, mom, a_start, a_end, steps, cosmo=None):
+        """Simulation::new + run: returns (pos, mom) after `steps` KDK steps (FP64 on the GPU)."""
+        cosmo = cosmo or Cosmology.lcdm_default()
+        pos = np.array(pos, dtype=np.float64, copy=True).reshape(-1, 3)
+        mom = np.array(mom, dtype=np.float64, copy=True).reshape(-1, 3)
+        self._check(self.lib.tpm_run(self.h, ctypes.byref(cosmo), _ptr(pos), _ptr(mom), len(pos), float(a_start),
+                                     float(a_end), int(steps)))
+        return pos, mom
+
+    def step_to_dev(self, d_pos, d_mom, n, a0, a1, d_acc, d_delta=0, stream=0, cosmo=None):
+        cosmo = cosmo or Cosmology.lcdm_default()
+        self._check(self.lib.tpm_step_to_dev(self.h, ctypes.byref(cosmo), ctypes.c_void_p(d_pos),
+                                             ctypes.c_void_p(d_mom), int(n), float(a0), float(a1),
+                                             ctypes.c_void_p(d_acc), ctypes.c_void_p(d_delta or None),
+                                             ctypes.c_void_p(stream or None)))
+
+    def sync(self):
+        self._check(self.lib.tpm_sync(self.h))
+
     def judge(self, pos, idx, a, theta=0.2):
         """FP64 TreePM force (FP64 PM + FP64 per-particle walk at `theta`) at caller indices idx."""
         pos = _f64(pos, (-1, 3))
@@ -303,3 +344,59 @@ def coupling(omega_m, box_size, a, n):
     f = np.float32
     l = f(box_size)
     return float(f(1.5) * f(omega_m) * (l * l * l) / (f(4.0) * f(np.pi) * f(a) * f(n)))
+
+
+def kdk_factors(a0, a1, cosmo=None):
+    """(am, k1, d, k2) of one KDK step (src/integrator.rs:174-186, 262-297), FP64."""
+    cosmo = cosmo or Cosmology.lcdm_default()
+    out = [ctypes.c_double() for _ in range(4)]
+    rc = load().tpm_kdk_factors(ctypes.byref(cosmo), float(a0), float(a1), *[ctypes.byref(o) for o in out])
+    if rc != TPM_OK:
+        raise TpmError(rc, "invalid KDK factor arguments")
+    return tuple(o.value for o in out)
+
+
+class DeviceArray:
+    """A device allocation made by libtpm's own HIP runtime (for the *_dev entry points)."""
+
+    def __init__(self, count, dtype):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        rc = load().tpm_device_alloc(self.nbytes, ctypes.byref(p))
+        if rc != TPM_OK:
+            raise TpmError(rc, f"device allocation of {self.nbytes} bytes failed")
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.count
+        rc = load().tpm_memcpy(ctypes.c_void_p(self.ptr), a.ctypes.data_as(ctypes.c_void_p), self.nbytes, 0)
+        if rc != TPM_OK:
+            raise TpmError(rc, "host to device copy failed")
+
+    def numpy(self):
+        out = np.empty(self.count, self.dtype)
+        rc = load().tpm_memcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr), self.nbytes, 1)
+        if rc != TPM_OK:
+            raise TpmError(rc, "device to host copy failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().tpm_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -154,6 +154,47 @@ int tpm_zeldovich(tpm_ctx *ctx, uint64_t seed, double k_c, double disp_rms, doub
 int tpm_zeldovich_dev(tpm_ctx *ctx, uint64_t seed, double k_c, double disp_rms, double mom_factor, double *d_pos,
                       double *d_mom, void *stream);
 
+/*
+ * Device memory helpers for the *_dev entry points (no reference counterpart: a host that keeps
+ * particles resident on the GPU needs them; they use the same HIP runtime as the kernels).
+ * kind: 0 host->device, 1 device->host, 2 device->device. tpm_sync waits for the context stream.
+ */
+int tpm_device_alloc(size_t bytes, void **out);
+int tpm_device_free(void *ptr);
+int tpm_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int tpm_sync(tpm_ctx *ctx);
+
+/* Background cosmology for the KDK step factors (src/cosmology.rs:9-34, flat: Or + Om + OL = 1). */
+typedef struct tpm_cosmology {
+    double omega_r;
+    double omega_m;
+    double omega_l;
+} tpm_cosmology;
+
+/*
+ * KDK step factors of Simulation::step_to (src/integrator.rs:174-186, 262-297) in FP64: the time
+ * midpoint am, the half-kick integrals k1 = int_{a0}^{am} da/(aE), k2 = int_{am}^{a1} da/(aE) and
+ * the drift integral d = int_{a0}^{a1} da/(a^3 E). No device work.
+ */
+int tpm_kdk_factors(const tpm_cosmology *cosmo, double a0, double a1, double *am, double *k1, double *d, double *k2);
+
+/*
+ * One KDK step, Simulation::step_to (src/integrator.rs:174-186), on device arrays: p += acc k1;
+ * x = (x + p d) mod L; acc = accelerations(a1) (written over d_acc); p += acc k2. d_acc must hold
+ * the accelerations at a0 on entry (e.g. from tpm_accelerations_dev_f64). Positions and momenta
+ * are FP64. d_delta (n_mesh^3) receives the density contrast at a1, or NULL.
+ */
+int tpm_step_to_dev(tpm_ctx *ctx, const tpm_cosmology *cosmo, double *d_pos, double *d_mom, size_t n, double a0,
+                    double a1, float *d_acc, float *d_delta, void *stream);
+
+/*
+ * Simulation::new followed by Simulation::run (src/integrator.rs:54-79, 165-171): the initial
+ * force at a_start, then `steps` KDK steps to a_end evenly spaced in ln a, on host arrays of FP64
+ * positions and momenta updated in place.
+ */
+int tpm_run(tpm_ctx *ctx, const tpm_cosmology *cosmo, double *pos, double *mom, size_t n, double a_start,
+            double a_end, int steps);
+
 #ifdef __cplusplus
 }
 #endif

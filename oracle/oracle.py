@@ -49,6 +49,13 @@ def lib():
             sig("ora_brute_force", None, [_P, _I64, R, R, R, R, R, _P])
             sig("ora_coupling", R, [R, R, R, _I64])
             sig("ora_accelerations", _I64, [_P, _I64, _I, R, R, R, _I, R, R, R, R, _P, _P])
+            sig("ora_cosmo_e", R, [R, R, R, R])
+            sig("ora_growth", R, [R, R, R, R])
+            sig("ora_growth_rate", R, [R, R, R, R])
+            sig("ora_kick_factor", R, [R, R, R, R, R])
+            sig("ora_drift_factor", R, [R, R, R, R, R])
+            sig("ora_midpoint", R, [R, R, R, R, R])
+            sig("ora_sim_run", None, [_P, _P, _I64, _I, R, R, R, R, R, R, _I, _I, R, R, R, R])
         L.ora_threads.restype = _I
         L.ora_set_threads.argtypes = [_I]
         _lib = L
@@ -205,3 +212,56 @@ def force_split(box_size, mesh_n, particles_per_side, softening_scale, theta):
     spacing = f(box_size) / f(particles_per_side)
     return {"r_split": float(r_split), "r_cut": float(f(4.5) * r_split),
             "softening": float(f(softening_scale) * spacing), "theta": float(theta)}
+
+
+# Cosmology::lcdm_default (src/cosmology.rs:21-30), evaluated in float32 like the reference.
+LCDM = {"omega_r": float(np.float32(9.14e-5)), "omega_m": float(np.float32(0.315)),
+        "omega_l": float(np.float32(1.0) - np.float32(0.315) - np.float32(9.14e-5))}
+
+
+def _cosmo(c):
+    c = c or LCDM
+    return c["omega_r"], c["omega_m"], c["omega_l"]
+
+
+def cosmo_e(a, cosmo=None, dtype=np.float32):
+    s, _ = _dt(dtype)
+    return getattr(lib(), "ora_cosmo_e" + s)(*_cosmo(cosmo), a)
+
+
+def growth(a, cosmo=None, dtype=np.float32):
+    """Cosmology::growth (src/cosmology.rs:82-85)."""
+    s, _ = _dt(dtype)
+    return getattr(lib(), "ora_growth" + s)(*_cosmo(cosmo), a)
+
+
+def growth_rate(a, cosmo=None, dtype=np.float32):
+    s, _ = _dt(dtype)
+    return getattr(lib(), "ora_growth_rate" + s)(*_cosmo(cosmo), a)
+
+
+def kick_factor(a0, a1, cosmo=None, dtype=np.float32):
+    s, _ = _dt(dtype)
+    return getattr(lib(), "ora_kick_factor" + s)(*_cosmo(cosmo), a0, a1)
+
+
+def drift_factor(a0, a1, cosmo=None, dtype=np.float32):
+    s, _ = _dt(dtype)
+    return getattr(lib(), "ora_drift_factor" + s)(*_cosmo(cosmo), a0, a1)
+
+
+def midpoint(a0, a1, cosmo=None, dtype=np.float32):
+    s, _ = _dt(dtype)
+    return getattr(lib(), "ora_midpoint" + s)(*_cosmo(cosmo), a0, a1)
+
+
+def sim_run(pos, mom, n, box_size, a_start, a_end, steps, split=None, cosmo=None, dtype=np.float32):
+    """Simulation::new + run (src/integrator.rs:54-79, 165-186); returns (pos, mom)."""
+    s, dt = _dt(dtype)
+    pos = np.array(pos, dtype=dt, copy=True).reshape(-1, 3)
+    mom = np.array(mom, dtype=dt, copy=True).reshape(-1, 3)
+    sp = split or {}
+    getattr(lib(), "ora_sim_run" + s)(_p(pos), _p(mom), len(pos), n, box_size, *_cosmo(cosmo), a_start, a_end,
+                                      steps, 1 if split else 0, sp.get("r_split", 0.0), sp.get("r_cut", 0.0),
+                                      sp.get("softening", 0.0), sp.get("theta", 0.0))
+    return pos, mom

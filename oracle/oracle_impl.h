@@ -538,6 +538,130 @@ int64_t F(ora_accelerations)(const REAL *pos, int64_t np, int n, REAL box_size, 
     return inter;
 }
 
+/* ------------------------------------------------- cosmology and KDK (cosmology.rs, integrator.rs) */
+
+/* cosmology.rs:32-34 E(a) = sqrt(Or a^-4 + Om a^-3 + OL) */
+REAL F(ora_cosmo_e)(REAL omr, REAL omm, REAL oml, REAL a) {
+    return sqrt(omr / (a * a * a * a) + omm / (a * a * a) + oml);
+}
+
+/* cosmology.rs:47-80 growth_state: RK4 in ln a, 4000 steps from a = 1e-3 (Meszaros start). */
+static void F(growth_state_)(REAL omr, REAL omm, REAL oml, REAL a_end, REAL *d_out, REAL *dl_out) {
+    const REAL a_start = (REAL)1e-3;
+    REAL d, dl;
+    if (omr > (REAL)0.0) {
+        REAL y = a_start * omm / omr;
+        d = (REAL)1.0 + (REAL)1.5 * y;
+        dl = (REAL)1.5 * y;
+    } else {
+        d = a_start;
+        dl = a_start;
+    }
+#define RHS_(la_, d_, dl_, od_, odl_)                                                                   \
+    do {                                                                                                \
+        REAL a_ = exp(la_);                                                                             \
+        REAL e2_ = omr / (a_ * a_ * a_ * a_) + omm / (a_ * a_ * a_) + oml;                              \
+        REAL dln_e_ = (REAL)0.5 * ((REAL)-4.0 * omr / (a_ * a_ * a_ * a_) - (REAL)3.0 * omm / (a_ * a_ * a_)) / e2_; \
+        REAL src_ = (REAL)1.5 * omm / (a_ * a_ * a_ * e2_) * (d_);                                      \
+        od_ = (dl_);                                                                                    \
+        odl_ = -((REAL)2.0 + dln_e_) * (dl_) + src_;                                                    \
+    } while (0)
+    const int steps = 4000;
+    REAL la0 = log(a_start);
+    REAL h = (log(a_end) - la0) / (REAL)steps;
+    for (int step = 0; step < steps; ++step) {
+        REAL la = la0 + (REAL)step * h;
+        REAL k1d, k1l, k2d, k2l, k3d, k3l, k4d, k4l;
+        RHS_(la, d, dl, k1d, k1l);
+        RHS_(la + (REAL)0.5 * h, d + (REAL)0.5 * h * k1d, dl + (REAL)0.5 * h * k1l, k2d, k2l);
+        RHS_(la + (REAL)0.5 * h, d + (REAL)0.5 * h * k2d, dl + (REAL)0.5 * h * k2l, k3d, k3l);
+        RHS_(la + h, d + h * k3d, dl + h * k3l, k4d, k4l);
+        d += h / (REAL)6.0 * (k1d + (REAL)2.0 * k2d + (REAL)2.0 * k3d + k4d);
+        dl += h / (REAL)6.0 * (k1l + (REAL)2.0 * k2l + (REAL)2.0 * k3l + k4l);
+    }
+#undef RHS_
+    *d_out = d;
+    *dl_out = dl;
+}
+
+/* cosmology.rs:82-91 growth (D(1) = 1) and growth_rate f = dlnD/dlna. */
+REAL F(ora_growth)(REAL omr, REAL omm, REAL oml, REAL a) {
+    REAL d, dl, d1, dl1;
+    F(growth_state_)(omr, omm, oml, a, &d, &dl);
+    F(growth_state_)(omr, omm, oml, (REAL)1.0, &d1, &dl1);
+    return d / d1;
+}
+
+REAL F(ora_growth_rate)(REAL omr, REAL omm, REAL oml, REAL a) {
+    REAL d, dl;
+    F(growth_state_)(omr, omm, oml, a, &d, &dl);
+    return dl / d;
+}
+
+/* integrator.rs:287-297 Simpson's rule with 64 intervals; kind 0: 1/(a E) (kick), 1: 1/(a^3 E) (drift). */
+static REAL F(integrate_)(REAL omr, REAL omm, REAL oml, REAL lo, REAL hi, int kind) {
+#define FN_(a_) (kind == 0 ? (REAL)1.0 / ((a_) * F(ora_cosmo_e)(omr, omm, oml, (a_))) \
+                           : (REAL)1.0 / ((a_) * (a_) * (a_) * F(ora_cosmo_e)(omr, omm, oml, (a_))))
+    const int n = 64;
+    REAL h = (hi - lo) / (REAL)n;
+    REAL sum = FN_(lo) + FN_(hi);
+    for (int i = 1; i < n; ++i) {
+        REAL weight = (i % 2 == 1) ? (REAL)4.0 : (REAL)2.0;
+        sum += weight * FN_(lo + (REAL)i * h);
+    }
+#undef FN_
+    return sum * h / (REAL)3.0;
+}
+
+REAL F(ora_kick_factor)(REAL omr, REAL omm, REAL oml, REAL a0, REAL a1) { return F(integrate_)(omr, omm, oml, a0, a1, 0); }
+REAL F(ora_drift_factor)(REAL omr, REAL omm, REAL oml, REAL a0, REAL a1) { return F(integrate_)(omr, omm, oml, a0, a1, 1); }
+
+/* integrator.rs:273-285 midpoint: 60 bisections for half the kick factor. */
+REAL F(ora_midpoint)(REAL omr, REAL omm, REAL oml, REAL a0, REAL a1) {
+    REAL half = F(ora_kick_factor)(omr, omm, oml, a0, a1) / (REAL)2.0;
+    REAL lo = a0, hi = a1;
+    for (int it = 0; it < 60; ++it) {
+        REAL mid = (REAL)0.5 * (lo + hi);
+        if (F(ora_kick_factor)(omr, omm, oml, a0, mid) < half)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return (REAL)0.5 * (lo + hi);
+}
+
+/* Rust's f32/f64 rem_euclid. */
+static inline REAL F(rem_euclid_)(REAL x, REAL l) {
+    REAL r = fmod(x, l);
+    return r < (REAL)0.0 ? r + fabs(l) : r;
+}
+
+/*
+ * integrator.rs:54-79 (Simulation::new computes the initial accelerations) and :165-186
+ * (run / step_to): `steps` KDK steps from a_start to a_end evenly spaced in ln a. pos and mom are
+ * updated in place. treepm / split as in ora_accelerations.
+ */
+void F(ora_sim_run)(REAL *pos, REAL *mom, int64_t np, int n, REAL box_size, REAL omr, REAL omm, REAL oml,
+                    REAL a_start, REAL a_end, int steps, int treepm, REAL r_split, REAL r_cut, REAL soft,
+                    REAL theta) {
+    REAL *acc = (REAL *)malloc(sizeof(REAL) * 3 * (np > 0 ? np : 1));
+    F(ora_accelerations)(pos, np, n, box_size, omm, a_start, treepm, r_split, r_cut, soft, theta, acc, NULL);
+    REAL a = a_start;
+    for (int i = 0; i < steps; ++i) {
+        REAL a1 = a_start * pow(a_end / a_start, (REAL)(i + 1) / (REAL)steps);
+        REAL am = F(ora_midpoint)(omr, omm, oml, a, a1);
+        REAL k1 = F(ora_kick_factor)(omr, omm, oml, a, am);
+        for (int64_t q = 0; q < 3 * np; ++q) mom[q] += acc[q] * k1;
+        REAL d = F(ora_drift_factor)(omr, omm, oml, a, a1);
+        for (int64_t q = 0; q < 3 * np; ++q) pos[q] = F(rem_euclid_)(pos[q] + mom[q] * d, box_size);
+        F(ora_accelerations)(pos, np, n, box_size, omm, a1, treepm, r_split, r_cut, soft, theta, acc, NULL);
+        REAL k2 = F(ora_kick_factor)(omr, omm, oml, am, a1);
+        for (int64_t q = 0; q < 3 * np; ++q) mom[q] += acc[q] * k2;
+        a = a1;
+    }
+    free(acc);
+}
+
 #undef F
 #undef CAT
 #undef CAT2

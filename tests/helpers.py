@@ -60,3 +60,28 @@ def lattice(n, box_size, centred=True, dtype=np.float32):
 
 # Cosmology of the reference (src/cosmology.rs:21-30).
 OMEGA_M = 0.315
+
+
+def plane_wave(n, box_size, amp, a_init, growth, growth_rate, e_of_a, dtype=np.float64):
+    """src/integrator.rs:326-348: lattice q = i h, Psi_x = -(A/k) sin(k q_x), x = q + D Psi,
+    p = a^2 f E D Psi. growth, growth_rate, e_of_a are callables of a."""
+    k = 2 * np.pi / box_size
+    h = box_size / n
+    d = growth(a_init)
+    f = growth_rate(a_init)
+    e = e_of_a(a_init)
+    i = np.arange(n)
+    qx, qy, qz = np.meshgrid(i * h, i * h, i * h, indexing="ij")
+    qx, qy, qz = qx.ravel(), qy.ravel(), qz.ravel()
+    psi = -(amp / k) * np.sin(k * qx)
+    x = np.mod(qx + d * psi, box_size)
+    pos = np.stack([x, qy, qz], 1).astype(dtype)
+    mom = np.stack([a_init * a_init * f * e * d * psi, 0 * qx, 0 * qx], 1).astype(dtype)
+    return pos, mom
+
+
+def mode_amplitude(delta, n, box_size):
+    """src/integrator.rs:351-366: 2 mean(delta cos(k x)) over cell centres."""
+    x = (np.arange(n) + 0.5) * box_size / n
+    c = np.cos(2 * np.pi / box_size * x)
+    return 2.0 * float((delta.reshape(n, n, n) * c[:, None, None]).sum()) / n ** 3
