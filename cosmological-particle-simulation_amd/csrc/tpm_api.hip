@@ -55,7 +55,7 @@ static void free_all(Ctx &c) {
     void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.sort_tmp, c.spos,
                     c.counter, c.flag, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
                     c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.scan_tmp,
-                    c.scan_tmp_storage, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp};
+                    c.scan_tmp_storage, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp, c.stab};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -96,7 +96,12 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     TPM_HIP(hipMalloc(&c.specC, c.hs * sizeof(float2)));
     TPM_HIP(hipMalloc(&c.force, 3 * c.m * sizeof(float)));
     TPM_HIP(hipMalloc(&c.delta, c.m * sizeof(float)));
+    TPM_HIP(hipMalloc(&c.stab, 1040 * sizeof(float2)));
+    int dev = 0;
+    TPM_HIP(hipGetDevice(&dev));
+    TPM_HIP(hipDeviceGetAttribute(&c.num_cu, hipDeviceAttributeMultiprocessorCount, dev));
     make_plan(c);
+    make_short_range_table(c);
 }
 
 // Stage timer: records an event on the context stream when timing is on.
@@ -289,6 +294,7 @@ int tpm_set_params(tpm_ctx *ctx, const tpm_params *params) {
         const bool box_changed = params->box_size != c.p.box_size;
         c.p = *params;
         if (box_changed) tpm::make_plan(c);
+        tpm::make_short_range_table(c);
         return TPM_OK;
     });
 }

@@ -111,6 +111,10 @@ struct Ctx {
     size_t judge_cap = 0;
 
     Tree tree;
+    float2 *stab = nullptr;     // S(u) table of the walk (kSTab + 1 entries)
+    double stab_scale = 0.0;
+    int num_cu = 0;
+    unsigned walk_blocks = 0;
     // Walk groups: particles [gstart[g], gstart[g+1]) of the Morton order.
     uint32_t *gstart = nullptr;     // max_n + 1
     uint32_t *gscratch = nullptr;   // 2 max_n words
@@ -158,6 +162,9 @@ void build_tree(Ctx &c, size_t n);
 // c.group_level (sets c.gstart, c.ngroups).
 void build_groups(Ctx &c, size_t n);
 size_t group_scan_bytes(size_t max_n);
+
+// S(u) table of the group walk for the context's r_split and r_cut.
+void make_short_range_table(Ctx &c);
 
 // Short-range force of every particle added into acc[idx[i]] (group walk or reference walk).
 void walk_short_range(Ctx &c, size_t n, double k, float *d_acc);

@@ -23,11 +23,15 @@
 //
 // Reference walk (TPM_FLAG_REF_WALK, and the FP64 judge): one thread per particle walking the
 // same octree with the reference's LIFO order and per-particle tests, in T = float or double.
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "tpm_internal.h"
 
 namespace tpm {
+
+constexpr int kSTab = 1024;  // intervals of the S(u) table on [0, u_cut]
 
 struct WalkArgs {
     const double *spos;
@@ -45,6 +49,10 @@ struct WalkArgs {
     float eps2;
     float inv2rs;
     float negk;
+    float tab_scale;  // (1 / (2 r_s)) / (u_cut / kSTab): sqrt(r^2) * tab_scale indexes the S table
+    const float2 *stab;
+    uint32_t *next_group;
+    uint32_t ngroups;
     float *acc;
     unsigned long long *counter;
 };
@@ -117,66 +125,116 @@ __device__ __forceinline__ double box_dist2(double dx, double dy, double dz, dou
     return ex * ex + ey * ey + ez * ez;
 }
 
+constexpr int kWaves = 4;  // wavefronts per workgroup; each walks its own groups
 constexpr int kStackCap = 768;
 constexpr int kStackAlloc = kStackCap + 192;
 constexpr int kRangeCap = 128;
 constexpr int kNodeCap = 128;
-constexpr int kRing = 128;  // staging ring of kept sources (power of two, >= 127)
+constexpr int kStage = 128;  // staged sources (kept ones accumulate here, evaluated 64 at a time)
 
-struct WalkShared {
+// Per-wavefront LDS state. The node tiles and the source stage are never live at the same time.
+struct WaveShared {
     uint32_t stack[kStackAlloc];
     uint32_t r_start[kRangeCap];
     uint32_t r_off[kRangeCap + 1];
     uint32_t nlist[kNodeCap];
-    float4 tileA[64];
-    float4 tileB[64];
-    float4 ring[kRing];
+    union {
+        float4 stage[kStage];
+        struct {
+            float4 tileA[64];
+            float4 tileB[64];
+        };
+    };
 };
 
-// Pairs of every active lane with `cnt` staged sources at ring positions head, head+1, ...
-// (Sources are broadcast from LDS; a lane-private compaction of the accepted pairs does not pay
-// here because consecutive sources are spatially coherent, so some lane accepts most of a tile.)
+struct BlockShared {
+    float2 stab[kSTab + 1];  // S(u) table: value and slope per interval
+    WaveShared wave[kWaves];
+};
+
+// Orders a wavefront's LDS writes before its later reads (one wavefront executes in lockstep, so
+// no hardware barrier is needed; this keeps the compiler from moving memory operations across).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Pair force with S(u) interpolated from the LDS table (linear in u over kSTab intervals of
+// [0, u_cut]; interpolation error below 1e-6). Used for particle pairs, which satisfy r <= r_cut.
+__device__ __forceinline__ void pair_tab(float dx, float dy, float dz, float r2, const float2 *stab,
+                                         const WalkArgs &w, float &ax, float &ay, float &az) {
+    const float s = r2 + w.eps2;
+    const float rs = __builtin_amdgcn_rsqf(s);
+    const float inv3 = rs * rs * rs;
+    const float x = __builtin_amdgcn_sqrtf(r2) * w.tab_scale;
+    const int i = min((int)x, kSTab - 1);
+    const float fr = x - (float)i;
+    const float2 t = stab[i];
+    const float f = w.negk * fmaf(fr, t.y, t.x) * inv3;
+    ax = fmaf(dx, f, ax);
+    ay = fmaf(dy, f, ay);
+    az = fmaf(dz, f, az);
+}
+
 template <bool WRAP, bool SELF>
-__device__ __forceinline__ void eval_ring(const WalkShared &sh, int head, int cnt, const WalkArgs &w, uint32_t self,
+__device__ __forceinline__ void stage_one(const float4 src, const float2 *stab, const WalkArgs &w, uint32_t self,
                                           float xi, float yi, float zi, bool active, float &ax, float &ay,
                                           float &az, uint32_t &ninter) {
-    if (!active) return;
-#pragma unroll 2
-    for (int s = 0; s < cnt; ++s) {
-        const float4 src = sh.ring[(head + s) & (kRing - 1)];
-        float dx = xi - src.x;
-        float dy = yi - src.y;
-        float dz = zi - src.z;
-        if (WRAP) {
-            dx -= w.boxf * rintf(dx * w.invboxf);
-            dy -= w.boxf * rintf(dy * w.invboxf);
-            dz -= w.boxf * rintf(dz * w.invboxf);
+    float dx = xi - src.x;
+    float dy = yi - src.y;
+    float dz = zi - src.z;
+    if (WRAP) {
+        dx -= w.boxf * rintf(dx * w.invboxf);
+        dy -= w.boxf * rintf(dy * w.invboxf);
+        dz -= w.boxf * rintf(dz * w.invboxf);
+    }
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    bool take = active && r2 <= w.cut2;
+    if (SELF) take = take && __float_as_uint(src.w) != self;
+    if (take) {
+        pair_tab(dx, dy, dz, r2, stab, w, ax, ay, az);
+        ++ninter;
+    }
+}
+
+// Pairs of every active lane with the first `cnt` staged sources (broadcast LDS reads).
+template <bool WRAP, bool SELF>
+__device__ __forceinline__ void eval_stage(const WaveShared &sh, const float2 *stab, int cnt, const WalkArgs &w,
+                                           uint32_t self, float xi, float yi, float zi, bool active, float &ax,
+                                           float &ay, float &az, uint32_t &ninter) {
+    if (cnt == 64) {
+        // Eight sources are read ahead into registers so their LDS latency overlaps.
+        for (int c0 = 0; c0 < 64; c0 += 8) {
+            float4 src[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) src[j] = sh.stage[c0 + j];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                stage_one<WRAP, SELF>(src[j], stab, w, self, xi, yi, zi, active, ax, ay, az, ninter);
         }
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        bool take = r2 <= w.cut2;
-        if (SELF) take = take && __float_as_uint(src.w) != self;
-        if (take) {
-            pair_f32(dx, dy, dz, r2, 1.0f, w, ax, ay, az);
-            ++ninter;
-        }
+    } else {
+        for (int s = 0; s < cnt; ++s)
+            stage_one<WRAP, SELF>(sh.stage[s], stab, w, self, xi, yi, zi, active, ax, ay, az, ninter);
     }
 }
 
 // Direct sums over the particles of the listed leaf ranges. Each source is first tested
 // against the group's bounding box (half extents hx, hy, hz about the group centre): one
 // farther than r_cut from the whole box cannot be within r_cut of any particle of the group,
-// so it is dropped before the 64-lane evaluation. Kept sources are compacted into an LDS ring
+// so it is dropped before the 64-lane evaluation. Kept sources are compacted into the stage
 // and evaluated 64 at a time.
 template <bool WRAP, bool SELF>
-__device__ __forceinline__ void eval_particles(WalkShared &sh, int nr, int tot, const WalkArgs &w, double gx,
-                                               double gy, double gz, float hx, float hy, float hz, uint32_t self,
-                                               float xi, float yi, float zi, bool active, float &ax, float &ay,
-                                               float &az, uint32_t &ninter, uint32_t &kept) {
-    const int lane = threadIdx.x;
+__device__ __forceinline__ void eval_particles(WaveShared &sh, const float2 *stab, int nr, int tot,
+                                               const WalkArgs &w, double gx, double gy, double gz, float hx,
+                                               float hy, float hz, uint32_t self, float xi, float yi, float zi,
+                                               bool active, float &ax, float &ay, float &az, uint32_t &ninter,
+                                               uint32_t &kept) {
+    const int lane = threadIdx.x & 63;
     if (lane == 0) sh.r_off[nr] = (uint32_t)tot;
-    __syncthreads();
+    wave_sync();
     const float cut2 = w.cut2 * (1.0f + 1e-5f);
-    int head = 0, cnt = 0;
+    int cnt = 0;
     for (int base = 0; base < tot; base += 64) {
         const int e = base + lane;
         bool keep = false;
@@ -210,32 +268,33 @@ __device__ __forceinline__ void eval_particles(WalkShared &sh, int nr, int tot, 
             }
         }
         const unsigned long long m = __ballot(keep);
-        if (keep) {
-            const int at = __popcll(m & ((1ull << lane) - 1ull));
-            sh.ring[(head + cnt + at) & (kRing - 1)] = v;
-        }
+        if (keep) sh.stage[cnt + __popcll(m & ((1ull << lane) - 1ull))] = v;
         cnt += __popcll(m);
-        __syncthreads();
+        wave_sync();
         if (cnt >= 64) {
-            eval_ring<WRAP, SELF>(sh, head, 64, w, self, xi, yi, zi, active, ax, ay, az, ninter);
-            head = (head + 64) & (kRing - 1);
-            cnt -= 64;
+            eval_stage<WRAP, SELF>(sh, stab, 64, w, self, xi, yi, zi, active, ax, ay, az, ninter);
             kept += 64;
-            __syncthreads();
+            const int rem = cnt - 64;
+            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lane < rem) t = sh.stage[64 + lane];
+            wave_sync();
+            if (lane < rem) sh.stage[lane] = t;
+            wave_sync();
+            cnt = rem;
         }
     }
     if (cnt > 0) {
-        eval_ring<WRAP, SELF>(sh, head, cnt, w, self, xi, yi, zi, active, ax, ay, az, ninter);
+        eval_stage<WRAP, SELF>(sh, stab, cnt, w, self, xi, yi, zi, active, ax, ay, az, ninter);
         kept += cnt;
-        __syncthreads();
+        wave_sync();
     }
 }
 
 template <bool WRAP>
-__device__ __forceinline__ void eval_nodes(WalkShared &sh, int nn, const WalkArgs &w, double gx, double gy, double gz,
-                                           float xi, float yi, float zi, bool active, float &ax, float &ay,
-                                           float &az, uint32_t &ninter) {
-    const int lane = threadIdx.x;
+__device__ __forceinline__ void eval_nodes(WaveShared &sh, int nn, const WalkArgs &w, double gx, double gy,
+                                           double gz, float xi, float yi, float zi, bool active, float &ax,
+                                           float &ay, float &az, uint32_t &ninter) {
+    const int lane = threadIdx.x & 63;
     const float sqrt3 = 1.7320508075688772f;
     for (int base = 0; base < nn; base += 64) {
         const int e = base + lane;
@@ -258,53 +317,53 @@ __device__ __forceinline__ void eval_nodes(WalkShared &sh, int nn, const WalkArg
             sh.tileA[lane] = make_float4((float)cx, (float)cy, (float)cz, (float)cm.w);
             sh.tileB[lane] = make_float4((float)bxx, (float)byy, (float)bzz, max_r * max_r);
         }
-        __syncthreads();
+        wave_sync();
         const int cnt = min(64, nn - base);
-        if (active) {
-            for (int s = 0; s < cnt; ++s) {
-                const float4 bb = sh.tileB[s];
-                float ux = bb.x - xi, uy = bb.y - yi, uz = bb.z - zi;
-                if (WRAP) {
-                    ux -= w.boxf * rintf(ux * w.invboxf);
-                    uy -= w.boxf * rintf(uy * w.invboxf);
-                    uz -= w.boxf * rintf(uz * w.invboxf);
-                }
-                if (fmaf(ux, ux, fmaf(uy, uy, uz * uz)) > bb.w) continue;
-                const float4 src = sh.tileA[s];
-                float dx = xi - src.x, dy = yi - src.y, dz = zi - src.z;
-                if (WRAP) {
-                    dx -= w.boxf * rintf(dx * w.invboxf);
-                    dy -= w.boxf * rintf(dy * w.invboxf);
-                    dz -= w.boxf * rintf(dz * w.invboxf);
-                }
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                pair_f32(dx, dy, dz, r2, src.w, w, ax, ay, az);
-                ++ninter;
+        for (int s = 0; s < cnt; ++s) {
+            const float4 bb = sh.tileB[s];
+            float ux = bb.x - xi, uy = bb.y - yi, uz = bb.z - zi;
+            if (WRAP) {
+                ux -= w.boxf * rintf(ux * w.invboxf);
+                uy -= w.boxf * rintf(uy * w.invboxf);
+                uz -= w.boxf * rintf(uz * w.invboxf);
             }
+            const bool inside = active && fmaf(ux, ux, fmaf(uy, uy, uz * uz)) <= bb.w;
+            if (!inside) continue;
+            ++ninter;
+            const float4 src = sh.tileA[s];
+            float dx = xi - src.x, dy = yi - src.y, dz = zi - src.z;
+            if (WRAP) {
+                dx -= w.boxf * rintf(dx * w.invboxf);
+                dy -= w.boxf * rintf(dy * w.invboxf);
+                dz -= w.boxf * rintf(dz * w.invboxf);
+            }
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            // A node's centre of mass may lie beyond r_cut, outside the table: analytic S(r).
+            pair_f32(dx, dy, dz, r2, src.w, w, ax, ay, az);
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
 template <bool WRAP, bool SELF>
-__device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, double gx, double gy, double gz, double hx,
-                                double hy, double hz, uint32_t self, float xi, float yi, float zi, bool active,
-                                float &ax, float &ay, float &az, uint32_t &ninter, uint32_t &listed,
-                                uint32_t &kept) {
+__device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkArgs &w, double gx, double gy,
+                                double gz, double hx, double hy, double hz, uint32_t self, float xi, float yi,
+                                float zi, bool active, float &ax, float &ay, float &az, uint32_t &ninter,
+                                uint32_t &listed, uint32_t &kept) {
     const float hxf = (float)hx * (1.0f + 1e-6f), hyf = (float)hy * (1.0f + 1e-6f), hzf = (float)hz * (1.0f + 1e-6f);
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const double sqrt3 = 1.7320508075688772;
     int size = 1;
     if (lane == 0) sh.stack[0] = 0;
     int nr = 0, tot = 0, nn = 0;
-    __syncthreads();
+    wave_sync();
     while (size > 0) {
         int room = (kStackCap - size) / 7;
         int k = min(64, size);
         if (k > room) k = room > 1 ? room : 1;
         uint32_t node = 0xFFFFFFFFu;
         if (lane < k) node = sh.stack[size - k + lane];
-        __syncthreads();
+        wave_sync();
         size -= k;
         int kind = 0;  // 0 none, 1 leaf range, 2 accept, 3 open
         uint2 rng = make_uint2(0, 0);
@@ -364,10 +423,10 @@ __device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, double gx, do
         nr += t_leaf;
         tot += t_leafcnt;
         nn += t_acc;
-        __syncthreads();
+        wave_sync();
         if (nr > kRangeCap - 64) {
-            eval_particles<WRAP, SELF>(sh, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active, ax, ay,
-                                       az, ninter, kept);
+            eval_particles<WRAP, SELF>(sh, stab, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active,
+                                       ax, ay, az, ninter, kept);
             listed += (uint32_t)tot;
             nr = 0;
             tot = 0;
@@ -378,18 +437,17 @@ __device__ void group_walk_body(WalkShared &sh, const WalkArgs &w, double gx, do
         }
     }
     if (nr > 0) {
-        eval_particles<WRAP, SELF>(sh, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active, ax, ay, az,
-                                   ninter, kept);
+        eval_particles<WRAP, SELF>(sh, stab, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active, ax,
+                                   ay, az, ninter, kept);
         listed += (uint32_t)tot;
     }
     if (nn > 0) eval_nodes<WRAP>(sh, nn, w, gx, gy, gz, xi, yi, zi, active, ax, ay, az, ninter);
 }
 
-__global__ void __launch_bounds__(64) k_group_walk(WalkArgs w) {
-    __shared__ WalkShared sh;
-    const int lane = threadIdx.x;
-    const size_t g0 = w.gstart[blockIdx.x];
-    const size_t g1 = w.gstart[blockIdx.x + 1];
+__device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w, uint32_t g) {
+    const int lane = threadIdx.x & 63;
+    const size_t g0 = w.gstart[g];
+    const size_t g1 = w.gstart[g + 1];
     const size_t i = g0 + lane;
     const bool active = i < g1;
     const size_t ii = active ? i : g0;
@@ -402,7 +460,7 @@ __global__ void __launch_bounds__(64) k_group_walk(WalkArgs w) {
     const double hx = 0.5 * (mxx - mnx) + pad, hy = 0.5 * (mxy - mny) + pad, hz = 0.5 * (mxz - mnz) + pad;
     const float xi = (float)(px - gx), yi = (float)(py - gy), zi = (float)(pz - gz);
     float ax = 0.f, ay = 0.f, az = 0.f;
-    uint32_t ninter = 0;
+    uint32_t ninter = 0, listed = 0, kept = 0;
     // Sources are placed at their image nearest the group centre; that image is the one nearest
     // each particle whenever half extent + r_cut < L/2 on every axis. Otherwise every pair and
     // node distance is wrapped individually.
@@ -410,21 +468,21 @@ __global__ void __launch_bounds__(64) k_group_walk(WalkArgs w) {
     const bool wrap = (hx + w.rcut >= lim) || (hy + w.rcut >= lim) || (hz + w.rcut >= lim);
     // With softening > 0 a particle's pair with itself has d = 0 and a finite kernel, so it adds
     // exactly zero (as the reference's j == i skip does); the index test is needed only at eps = 0.
-    uint32_t listed = 0, kept = 0;
     const bool self_test = !(w.eps2 > 0.0f);
+    const uint32_t self = (uint32_t)i;
     if (wrap) {
         if (self_test)
-            group_walk_body<true, true>(sh, w, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay, az,
+            group_walk_body<true, true>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay, az,
                                         ninter, listed, kept);
         else
-            group_walk_body<true, false>(sh, w, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay, az,
+            group_walk_body<true, false>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay, az,
                                          ninter, listed, kept);
     } else {
         if (self_test)
-            group_walk_body<false, true>(sh, w, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay, az,
-                                         ninter, listed, kept);
+            group_walk_body<false, true>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay,
+                                         az, ninter, listed, kept);
         else
-            group_walk_body<false, false>(sh, w, gx, gy, gz, hx, hy, hz, (uint32_t)i, xi, yi, zi, active, ax, ay,
+            group_walk_body<false, false>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay,
                                           az, ninter, listed, kept);
     }
     if (active) {
@@ -433,12 +491,29 @@ __global__ void __launch_bounds__(64) k_group_walk(WalkArgs w) {
         w.acc[3 * o + 1] += ay;
         w.acc[3 * o + 2] += az;
     }
-    int tot;
-    wave_excl_scan(active ? (int)ninter : 0, tot);
+    int ntot;
+    wave_excl_scan(active ? (int)ninter : 0, ntot);
     if (lane == 0 && w.counter) {
-        atomicAdd(w.counter, (unsigned long long)tot);
+        atomicAdd(w.counter, (unsigned long long)ntot);
         atomicAdd(w.counter + 2, (unsigned long long)listed);
         atomicAdd(w.counter + 3, (unsigned long long)kept);
+    }
+}
+
+// Persistent workgroups of kWaves wavefronts; each wavefront takes the next group from a global
+// counter until all groups are done (every wavefront reaches the exit test).
+__global__ void __launch_bounds__(64 * kWaves) k_group_walk(WalkArgs w) {
+    __shared__ BlockShared bs;
+    for (int i = threadIdx.x; i <= kSTab; i += blockDim.x) bs.stab[i] = w.stab[i];
+    __syncthreads();
+    WaveShared &sh = bs.wave[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        uint32_t g = 0;
+        if (lane == 0) g = atomicAdd(w.next_group, 1u);
+        g = __shfl(g, 0, 64);
+        if (g >= w.ngroups) break;
+        walk_group(sh, bs.stab, w, g);
     }
 }
 
@@ -580,12 +655,43 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc) {
     w.eps2 = (float)c.p.softening * (float)c.p.softening;
     w.inv2rs = (float)(1.0 / (2.0 * c.p.r_split));
     w.negk = (float)(-k);
+    w.tab_scale = (float)(c.stab_scale);
+    w.stab = c.stab;
     w.acc = d_acc;
     w.counter = c.counter;
     build_groups(c, n);
     w.gstart = c.gstart;
-    hipLaunchKernelGGL(k_group_walk, dim3((unsigned)c.ngroups), dim3(kGroup), 0, c.stream, w);
+    w.ngroups = (uint32_t)c.ngroups;
+    w.next_group = reinterpret_cast<uint32_t *>(c.flag) + 2;
+    TPM_HIP(hipMemsetAsync(w.next_group, 0, sizeof(uint32_t), c.stream));
+    if (c.walk_blocks == 0) {
+        int per_cu = 0;
+        TPM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_walk, 64 * kWaves, 0));
+        c.walk_blocks = (unsigned)std::max(1, per_cu) * (unsigned)c.num_cu;
+    }
+    const unsigned nblk = (unsigned)std::min<size_t>(c.walk_blocks, (c.ngroups + kWaves - 1) / kWaves);
+    if (nblk == 0) return;
+    hipLaunchKernelGGL(k_group_walk, dim3(nblk), dim3(64 * kWaves), 0, c.stream, w);
     TPM_HIP(hipGetLastError());
+}
+
+// S(u) = erfc(u) + (2/sqrt(pi)) u exp(-u^2) (src/tree.rs:54-57) tabulated in FP64 on
+// [0, u_cut (1 + 1e-4)], u_cut = r_cut / (2 r_s): kSTab + 1 points, value and slope per interval.
+void make_short_range_table(Ctx &c) {
+    if (!(c.p.r_split > 0) || !(c.p.r_cut > 0)) {
+        c.stab_scale = 0.0;
+        return;
+    }
+    const double umax = c.p.r_cut / (2.0 * c.p.r_split) * (1.0 + 1e-4);
+    const double du = umax / kSTab;
+    auto S = [](double u) { return std::erfc(u) + 1.1283791670955126 * u * std::exp(-u * u); };
+    std::vector<float2> t(kSTab + 1);
+    for (int i = 0; i <= kSTab; ++i) {
+        const double v0 = S(i * du), v1 = S((i + 1) * du);
+        t[i] = make_float2((float)v0, (float)(v1 - v0));
+    }
+    TPM_HIP(hipMemcpy(c.stab, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
+    c.stab_scale = 1.0 / (2.0 * c.p.r_split) / du;
 }
 
 void walk_judge_f64(Ctx &c, size_t n, const uint32_t *d_targets_sorted, size_t m, double k, double theta,
