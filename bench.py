@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=512, help="particles per side (= mesh cells per side)")
+    ap.add_argument("--workload", choices=["zeldovich", "nfw"], default="zeldovich",
+                    help="zeldovich: lattice ICs at z = 49; nfw: 20,000 NFW halos + 30%% background at z = 0")
     ap.add_argument("--theta", type=float, default=0.35)
     ap.add_argument("--judge-sample", type=int, default=16384)
     ap.add_argument("--no-judge", action="store_true")
@@ -79,14 +81,17 @@ def main():
 
     n = args.size
     npart = n ** 3
-    a = 1.0 / (1.0 + Z_INIT)
+    a = 1.0 / (1.0 + Z_INIT) if args.workload == "zeldovich" else 1.0
     split = treepm.force_split(BOX, n, n, 1 / 30, args.theta)
     log(f"[bench] {npart} particles, mesh {n}^3, theta {args.theta}")
 
     solver = treepm.Solver(n, BOX, OMEGA_M, split=split, max_particles=npart)
     pos = treepm.DeviceArray(3 * npart, np.float64)
     acc = treepm.DeviceArray(3 * npart, np.float32)
-    solver.zeldovich_dev(1, K_C, 0.3 * BOX / n, 0.0, pos.ptr)
+    if args.workload == "zeldovich":
+        solver.zeldovich_dev(1, K_C, 0.3 * BOX / n, 0.0, pos.ptr)
+    else:
+        solver.nfw_halos_dev(treepm.NfwSpec.clustered(seed=1), npart, pos.ptr)
     solver.sync()
     log("[bench] ICs generated")
 
@@ -130,7 +135,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"treepm_zeldovich_{n}cubed_mesh{n}_z49", "particles": npart, "mesh": n,
+        "config": {"workload": (f"treepm_zeldovich_{n}cubed_mesh{n}_z49" if args.workload == "zeldovich"
+                                else f"treepm_nfw20000_{n}cubed_mesh{n}_z0"), "particles": npart, "mesh": n,
                    "box": BOX, "theta": args.theta, "r_split_cells": 1.25, "r_cut_over_r_split": 4.5,
                    "softening": split["softening"], "positions": "f64", "pair_and_mesh_arithmetic": "f32"},
         "stages_ms": stages,

@@ -8,7 +8,10 @@
 // power law P(k) = k^-2 exp(-k / k_c), and its amplitude is set by rescaling Psi to a target
 // 3D displacement RMS. The noise is a counter-based hash (SplitMix64 finaliser, Box-Muller),
 // so a grid point's value depends only on (seed, index) and can be restated off the GPU.
+#include <algorithm>
 #include <cmath>
+#include <utility>
+#include <vector>
 
 #include "tpm_internal.h"
 
@@ -102,6 +105,129 @@ void zeldovich(Ctx &c, uint64_t seed, double kc, double disp_rms, double mom_fac
     hipLaunchKernelGGL(k_place, dim3(blocks(c.m, 256)), dim3(256), 0, c.stream, c.dforce, c.n, c.p.box_size, scale,
                        mom_factor, d_pos, d_mom);
     TPM_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ clustered NFW particles
+
+// One halo of the clustered workload: centre, r200, NFW concentration and first particle.
+struct HaloDev {
+    double cx, cy, cz, r200, conc;
+    uint64_t first;
+};
+
+__device__ inline double u01(uint64_t seed, uint64_t idx, uint64_t stream) {
+    const uint64_t x = splitmix64(seed ^ splitmix64(idx * 4 + stream));
+    return ((double)(x >> 11) + 0.5) * 0x1p-53;  // (0, 1)
+}
+
+// NFW enclosed-mass shape m(x) = ln(1 + x) - x / (1 + x), x = r / r_s.
+__device__ inline double nfw_m(double x) { return log1p(x) - x / (1.0 + x); }
+
+// Particle i: halo particles (i < n_halo_part) sample r from the NFW mass profile truncated at
+// r200 (bisection on m(x) = u m(c)) in an isotropic direction; the rest are uniform.
+__global__ void k_nfw_place(const HaloDev *__restrict__ halos, int nh, size_t n_halo_part, size_t n, double box,
+                            uint64_t seed, double *__restrict__ pos) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double p[3];
+    if (i < n_halo_part) {
+        int lo = 0, hi = nh - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (halos[mid].first <= i)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const HaloDev h = halos[lo];
+        const double target = u01(seed, i, 0) * nfw_m(h.conc);
+        double a = 0.0, b = h.conc;
+        for (int it = 0; it < 60; ++it) {
+            const double mid = 0.5 * (a + b);
+            if (nfw_m(mid) < target)
+                a = mid;
+            else
+                b = mid;
+        }
+        const double r = 0.5 * (a + b) * h.r200 / h.conc;
+        const double cth = 2.0 * u01(seed, i, 1) - 1.0;
+        const double sth = sqrt(fmax(0.0, 1.0 - cth * cth));
+        const double ph = 6.283185307179586 * u01(seed, i, 2);
+        p[0] = h.cx + r * sth * cos(ph);
+        p[1] = h.cy + r * sth * sin(ph);
+        p[2] = h.cz + r * cth;
+    } else {
+        for (int a = 0; a < 3; ++a) p[a] = box * u01(seed, i, (uint64_t)a);
+    }
+    for (int a = 0; a < 3; ++a) pos[3 * i + a] = rem_euclid(p[a], box);
+}
+
+// Host-side halo catalogue (counter-based, so it is reproducible from the seed alone).
+static double host_u01(uint64_t seed, uint64_t idx, uint64_t stream) {
+    const uint64_t x = splitmix64(seed ^ splitmix64(idx * 4 + stream));
+    return ((double)(x >> 11) + 0.5) * 0x1p-53;
+}
+
+void nfw_halos(Ctx &c, const NfwSpec &sp, size_t n, double *d_pos) {
+    TPM_REQUIRE(n <= c.max_n, "more particles than the context capacity");
+    TPM_REQUIRE(sp.n_halos > 0 && sp.halo_fraction >= 0 && sp.halo_fraction <= 1, "bad halo count or fraction");
+    TPM_REQUIRE(sp.mass_min > 0 && sp.mass_max > sp.mass_min && sp.slope != 1.0, "bad halo mass range or slope");
+    TPM_REQUIRE(sp.conc_min > 0 && sp.conc_max >= sp.conc_min, "bad concentration range");
+    const double box = c.p.box_size;
+    const int nh = sp.n_halos;
+    const uint64_t hseed = splitmix64(sp.seed ^ 0xA5A5A5A5A5A5A5A5ULL);
+    // Masses from dN/dM ~ M^-slope on [mass_min, mass_max] by inverse transform.
+    const double e = 1.0 - sp.slope;
+    const double a0 = std::pow(sp.mass_min, e), a1 = std::pow(sp.mass_max, e);
+    std::vector<double> mass(nh);
+    double mtot = 0.0;
+    for (int h = 0; h < nh; ++h) {
+        mass[h] = std::pow(a0 + host_u01(hseed, h, 0) * (a1 - a0), 1.0 / e);
+        mtot += mass[h];
+    }
+    // Particles per halo in proportion to mass (largest remainders get the leftovers).
+    const size_t nhp = (size_t)std::llround(sp.halo_fraction * (double)n);
+    std::vector<size_t> cnt(nh);
+    std::vector<std::pair<double, int>> rem(nh);
+    size_t used = 0;
+    for (int h = 0; h < nh; ++h) {
+        const double want = mass[h] / mtot * (double)nhp;
+        cnt[h] = (size_t)std::floor(want);
+        used += cnt[h];
+        rem[h] = {want - std::floor(want), h};
+    }
+    std::sort(rem.begin(), rem.end(), [](const std::pair<double, int> &x, const std::pair<double, int> &y) {
+        return x.first > y.first || (x.first == y.first && x.second < y.second);
+    });
+    for (size_t k = 0; used < nhp; ++k, ++used) cnt[rem[k % nh].second]++;
+    // r200 = (3 M / (4 pi 200 rho_crit))^(1/3), rho_crit = 2.775e11 (M_sun/h) / (Mpc/h)^3.
+    const double rho_crit = 2.77536627e11;
+    std::vector<HaloDev> hd(nh);
+    uint64_t first = 0;
+    for (int h = 0; h < nh; ++h) {
+        hd[h].cx = box * host_u01(hseed, h, 1);
+        hd[h].cy = box * host_u01(hseed, h, 2);
+        hd[h].cz = box * host_u01(hseed, h, 3);
+        hd[h].r200 = std::cbrt(3.0 * mass[h] / (4.0 * M_PI * 200.0 * rho_crit));
+        hd[h].conc = sp.conc_min + (sp.conc_max - sp.conc_min) * host_u01(hseed + 1, h, 0);
+        hd[h].first = first;
+        first += cnt[h];
+    }
+    HaloDev *d_h = nullptr;
+    TPM_HIP(hipMalloc(&d_h, nh * sizeof(HaloDev)));
+    try {
+        TPM_HIP(hipMemcpyAsync(d_h, hd.data(), nh * sizeof(HaloDev), hipMemcpyHostToDevice, c.stream));
+        if (n) {
+            hipLaunchKernelGGL(k_nfw_place, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_h, nh, nhp, n, box,
+                               splitmix64(sp.seed), d_pos);
+            TPM_HIP(hipGetLastError());
+        }
+        TPM_HIP(hipStreamSynchronize(c.stream));
+    } catch (...) {
+        (void)hipFree(d_h);
+        throw;
+    }
+    (void)hipFree(d_h);
 }
 
 }  // namespace tpm

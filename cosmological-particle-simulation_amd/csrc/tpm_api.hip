@@ -572,6 +572,31 @@ int tpm_run(tpm_ctx *ctx, const tpm_cosmology *cosmo, double *pos, double *mom, 
     });
 }
 
+int tpm_nfw_halos_dev(tpm_ctx *ctx, const tpm_nfw_spec *spec, size_t n, double *d_pos, void *stream) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(spec != nullptr && (n == 0 || d_pos != nullptr), "null argument");
+        hipStream_t saved = c.stream;
+        if (stream) c.stream = (hipStream_t)stream;
+        try {
+            tpm::nfw_halos(c, *spec, n, d_pos);
+        } catch (...) {
+            c.stream = saved;
+            throw;
+        }
+        c.stream = saved;
+        return TPM_OK;
+    });
+}
+
+int tpm_nfw_halos(tpm_ctx *ctx, const tpm_nfw_spec *spec, size_t n, double *pos) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(spec != nullptr && (n == 0 || pos != nullptr), "null argument");
+        tpm::nfw_halos(c, *spec, n, c.pos);
+        if (n) TPM_HIP(hipMemcpy(pos, c.pos, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+        return TPM_OK;
+    });
+}
+
 int tpm_zeldovich_dev(tpm_ctx *ctx, uint64_t seed, double k_c, double disp_rms, double mom_factor, double *d_pos,
                       double *d_mom, void *stream) {
     return guarded(ctx, [&](Ctx &c) {

@@ -194,6 +194,11 @@ KdkFactors kdk_factors(const tpm_cosmology &c, double a0, double a1);
 void kick(Ctx &c, double *d_mom, const float *d_acc, size_t n, double f);
 void drift(Ctx &c, double *d_pos, const double *d_mom, size_t n, double f);
 
+// Clustered workload (ic_kernels.hip): halo particles from NFW profiles plus a uniform
+// background, positions written to d_pos (3 n doubles).
+void nfw_halos(Ctx &c, const tpm_nfw_spec &sp, size_t n, double *d_pos);
+using NfwSpec = tpm_nfw_spec;
+
 // Plans and tables (fft_kernels.hip).
 void make_plan(Ctx &c);
 

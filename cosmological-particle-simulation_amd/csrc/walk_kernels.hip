@@ -160,18 +160,18 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Pair force with S(u) interpolated from the LDS table (linear in u over kSTab intervals of
-// [0, u_cut]; interpolation error below 1e-6). Used for particle pairs, which satisfy r <= r_cut.
+// Pair force with -K S(u) interpolated from the LDS table (linear in u over kSTab intervals of
+// [0, u_cut (1 + 1e-4)]; interpolation error below 1e-6). Used for particle pairs, which satisfy
+// r <= r_cut, so the interval index is at most kSTab - 1 and needs no clamp.
 __device__ __forceinline__ void pair_tab(float dx, float dy, float dz, float r2, const float2 *stab,
                                          const WalkArgs &w, float &ax, float &ay, float &az) {
     const float s = r2 + w.eps2;
     const float rs = __builtin_amdgcn_rsqf(s);
     const float inv3 = rs * rs * rs;
     const float x = __builtin_amdgcn_sqrtf(r2) * w.tab_scale;
-    const int i = min((int)x, kSTab - 1);
-    const float fr = x - (float)i;
-    const float2 t = stab[i];
-    const float f = w.negk * fmaf(fr, t.y, t.x) * inv3;
+    const float fr = __builtin_amdgcn_fractf(x);
+    const float2 t = stab[(int)x];
+    const float f = fmaf(fr, t.y, t.x) * inv3;
     ax = fmaf(dx, f, ax);
     ay = fmaf(dy, f, ay);
     az = fmaf(dz, f, az);
@@ -504,7 +504,11 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
 // counter until all groups are done (every wavefront reaches the exit test).
 __global__ void __launch_bounds__(64 * kWaves) k_group_walk(WalkArgs w) {
     __shared__ BlockShared bs;
-    for (int i = threadIdx.x; i <= kSTab; i += blockDim.x) bs.stab[i] = w.stab[i];
+    // The table holds -K S(u) for this evaluation's coupling K.
+    for (int i = threadIdx.x; i <= kSTab; i += blockDim.x) {
+        const float2 t = w.stab[i];
+        bs.stab[i] = make_float2(w.negk * t.x, w.negk * t.y);
+    }
     __syncthreads();
     WaveShared &sh = bs.wave[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;

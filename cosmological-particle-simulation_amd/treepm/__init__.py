@@ -42,7 +42,7 @@ EXPORTS = [
     "tpm_density_contrast_f32", "tpm_pm_accel", "tpm_interpolate_f32",
     "tpm_short_range_f32", "tpm_short_range_f64", "tpm_judge_f64",
     "tpm_zeldovich", "tpm_zeldovich_dev", "tpm_kdk_factors", "tpm_step_to_dev", "tpm_run",
-    "tpm_device_alloc", "tpm_device_free", "tpm_memcpy", "tpm_sync",
+    "tpm_device_alloc", "tpm_device_free", "tpm_memcpy", "tpm_sync", "tpm_nfw_halos", "tpm_nfw_halos_dev",
 ]
 
 
@@ -64,6 +64,21 @@ class Cosmology(ctypes.Structure):
         c.omega_m = float(f(0.315))
         c.omega_l = float(f(1.0) - f(0.315) - f(9.14e-5))
         return c
+
+
+class NfwSpec(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("n_halos", ctypes.c_int32), ("halo_fraction", ctypes.c_double),
+                ("mass_min", ctypes.c_double), ("mass_max", ctypes.c_double), ("slope", ctypes.c_double),
+                ("conc_min", ctypes.c_double), ("conc_max", ctypes.c_double)]
+
+    @classmethod
+    def clustered(cls, seed=1, n_halos=20000, halo_fraction=0.7):
+        """The clustered stress workload: 20,000 halos, M^-1.9 on [1e11, 1e15], c in [5, 15], 30% background."""
+        s = cls()
+        s.seed, s.n_halos, s.halo_fraction = int(seed), int(n_halos), float(halo_fraction)
+        s.mass_min, s.mass_max, s.slope = 1e11, 1e15, 1.9
+        s.conc_min, s.conc_max = 5.0, 15.0
+        return s
 
 
 class Params(ctypes.Structure):
@@ -139,6 +154,8 @@ def load():
         "tpm_device_free": (ctypes.c_int, [P]),
         "tpm_memcpy": (ctypes.c_int, [P, P, S, ctypes.c_int]),
         "tpm_sync": (ctypes.c_int, [P]),
+        "tpm_nfw_halos": (ctypes.c_int, [P, ctypes.POINTER(NfwSpec), S, P]),
+        "tpm_nfw_halos_dev": (ctypes.c_int, [P, ctypes.POINTER(NfwSpec), S, P, P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -325,6 +342,15 @@ class Solver:
                                              ctypes.c_void_p(d_mom), int(n), float(a0), float(a1),
                                              ctypes.c_void_p(d_acc), ctypes.c_void_p(d_delta or None),
                                              ctypes.c_void_p(stream or None)))
+
+    def nfw_halos(self, spec, n):
+        pos = np.empty((int(n), 3), np.float64)
+        self._check(self.lib.tpm_nfw_halos(self.h, ctypes.byref(spec), int(n), _ptr(pos)))
+        return pos
+
+    def nfw_halos_dev(self, spec, n, d_pos, stream=0):
+        self._check(self.lib.tpm_nfw_halos_dev(self.h, ctypes.byref(spec), int(n), ctypes.c_void_p(d_pos),
+                                               ctypes.c_void_p(stream or None)))
 
     def sync(self):
         self._check(self.lib.tpm_sync(self.h))

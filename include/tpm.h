@@ -164,6 +164,25 @@ int tpm_device_free(void *ptr);
 int tpm_memcpy(void *dst, const void *src, size_t bytes, int kind);
 int tpm_sync(tpm_ctx *ctx);
 
+/*
+ * Clustered particle set (no reference counterpart; the tree walk's stress workload): n_halos
+ * halos with masses from dN/dM ~ M^-slope on [mass_min, mass_max] (M_sun/h), NFW concentrations
+ * uniform in [conc_min, conc_max], centres uniform in the box, particles placed isotropically
+ * inside r200 (rho_crit = 2.775e11 h^2 M_sun/Mpc^3, lengths in the box's Mpc/h units) in numbers
+ * proportional to mass, halo_fraction of all particles; the rest uniform. Counter-based
+ * randomness: the output depends only on the spec and n.
+ */
+typedef struct tpm_nfw_spec {
+    uint64_t seed;
+    int32_t n_halos;
+    double halo_fraction;
+    double mass_min, mass_max, slope;
+    double conc_min, conc_max;
+} tpm_nfw_spec;
+
+int tpm_nfw_halos(tpm_ctx *ctx, const tpm_nfw_spec *spec, size_t n, double *pos);
+int tpm_nfw_halos_dev(tpm_ctx *ctx, const tpm_nfw_spec *spec, size_t n, double *d_pos, void *stream);
+
 /* Background cosmology for the KDK step factors (src/cosmology.rs:9-34, flat: Or + Om + OL = 1). */
 typedef struct tpm_cosmology {
     double omega_r;
