@@ -7,8 +7,12 @@
 
 #include "tpm_internal.h"
 
+#include <algorithm>
+
 namespace tpm {
 size_t sort_temp_bytes(size_t max_n);
+hipError_t rocprim_exclusive_scan_u32(void *tmp, size_t &bytes, const uint32_t *in, uint32_t *out, size_t n,
+                                      hipStream_t s);
 size_t group_scan_bytes(size_t max_n);
 
 static std::string g_create_error;
@@ -55,7 +59,14 @@ static void free_all(Ctx &c) {
     void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.sort_tmp, c.spos,
                     c.counter, c.flag, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
                     c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.scan_tmp,
-                    c.scan_tmp_storage, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp, c.stab};
+                    c.scan_tmp_storage, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp, c.stab,
+                    c.acc_sr, c.members, c.reject, c.mlp.params, c.mlp.grads, c.mlp.m1, c.mlp.m2};
+    for (int l = 0; l < 4; ++l) {
+        if (c.mlp.wt16[l]) (void)hipFree(c.mlp.wt16[l]);
+        if (c.mlp.w16[l]) (void)hipFree(c.mlp.w16[l]);
+    }
+    for (void *p : c.mlp.work)
+        if (p) (void)hipFree(p);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -497,6 +508,240 @@ int tpm_memcpy(void *dst, const void *src, size_t bytes, int kind) {
 int tpm_sync(tpm_ctx *ctx) {
     return guarded(ctx, [&](Ctx &c) {
         TPM_HIP(hipStreamSynchronize(c.stream));
+        return TPM_OK;
+    });
+}
+
+size_t tpm_mlp_param_count(void) { return tpm::mlp_param_count(); }
+
+int tpm_mlp_init(tpm_ctx *ctx, uint64_t seed) {
+    return guarded(ctx, [&](Ctx &c) {
+        tpm::mlp_init(c, seed);
+        c.mlp.trained = false;
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        return TPM_OK;
+    });
+}
+
+int tpm_mlp_get_params(tpm_ctx *ctx, float *params, size_t count) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(params && count == tpm::mlp_param_count(), "params must hold tpm_mlp_param_count() floats");
+        TPM_REQUIRE(c.mlp.params != nullptr, "the MLP is not initialised");
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        TPM_HIP(hipMemcpy(params, c.mlp.params, count * sizeof(float), hipMemcpyDeviceToHost));
+        return TPM_OK;
+    });
+}
+
+int tpm_mlp_set_params(tpm_ctx *ctx, const float *params, size_t count) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(params && count == tpm::mlp_param_count(), "params must hold tpm_mlp_param_count() floats");
+        if (!c.mlp.params) tpm::mlp_init(c, 1);
+        TPM_HIP(hipMemcpy(c.mlp.params, params, count * sizeof(float), hipMemcpyHostToDevice));
+        tpm::mlp_refresh_bf16(c);
+        c.mlp.trained = true;
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        return TPM_OK;
+    });
+}
+
+static size_t round32(size_t r) { return (r + 31) / 32 * 32; }
+
+int tpm_mlp_gradients(tpm_ctx *ctx, const float *x, const float *t, size_t rows, float *grads, double *loss) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(x && t && grads && rows > 0, "null argument");
+        TPM_REQUIRE(c.mlp.params != nullptr, "the MLP is not initialised");
+        const size_t rp = round32(rows);
+        tpm::mlp_prepare(c, rp);
+        float *dx = nullptr;
+        TPM_HIP(hipMalloc(&dx, rows * 36 * sizeof(float)));
+        try {
+            TPM_HIP(hipMemcpyAsync(dx, x, rows * 33 * sizeof(float), hipMemcpyHostToDevice, c.stream));
+            TPM_HIP(hipMemcpyAsync(dx + rows * 33, t, rows * 3 * sizeof(float), hipMemcpyHostToDevice, c.stream));
+            tpm::mlp_pack_rows(c, dx, dx + rows * 33, rows, rp);
+            const double l = tpm::mlp_forward_backward(c, rp, rows, true);
+            if (loss) *loss = l;
+            TPM_HIP(hipMemcpy(grads, c.mlp.grads, tpm::mlp_param_count() * sizeof(float), hipMemcpyDeviceToHost));
+        } catch (...) {
+            (void)hipFree(dx);
+            throw;
+        }
+        (void)hipFree(dx);
+        return TPM_OK;
+    });
+}
+
+int tpm_mlp_predict(tpm_ctx *ctx, const float *x, size_t rows, float *out) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(x && out && rows > 0, "null argument");
+        TPM_REQUIRE(c.mlp.params != nullptr, "the MLP is not initialised");
+        const size_t rp = round32(rows);
+        tpm::mlp_prepare(c, rp);
+        float *dx = nullptr;
+        TPM_HIP(hipMalloc(&dx, rows * 33 * sizeof(float)));
+        try {
+            TPM_HIP(hipMemcpyAsync(dx, x, rows * 33 * sizeof(float), hipMemcpyHostToDevice, c.stream));
+            tpm::mlp_pack_rows(c, dx, nullptr, rows, rp);
+            tpm::mlp_forward(c, rp, false);
+            TPM_HIP(hipStreamSynchronize(c.stream));
+            std::vector<float> z(rp * 32);
+            TPM_HIP(hipMemcpy(z.data(), c.mlp.Zout, rp * 32 * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t r = 0; r < rows; ++r)
+                for (int k = 0; k < 4; ++k) out[4 * r + k] = z[32 * r + k];
+        } catch (...) {
+            (void)hipFree(dx);
+            throw;
+        }
+        (void)hipFree(dx);
+        return TPM_OK;
+    });
+}
+
+// K / h^2: the force scale of the learned model's inputs and outputs.
+static double learned_fscale(const Ctx &c, double a, size_t n) {
+    const double k = 1.5 * c.p.omega_m * c.p.box_size * c.p.box_size * c.p.box_size / (4.0 * M_PI * a * (double)n);
+    const double h = c.p.box_size / c.n;
+    return k / (h * h);
+}
+
+// PM part of an evaluation with the density contrast kept in c.delta: sort, deposit, r2c ... c2r,
+// interpolation into d_acc.
+static void pm_part(Ctx &c, const double *d_pos, size_t n, double a, float *d_acc) {
+    tpm::sort_particles(c, d_pos, n);
+    tpm::deposit_fixed(c, n, 1.0, false, c.spos, true);
+    tpm::pm_from_grid(c, n, a, c.p.r_split, c.delta);
+    tpm::interp_force(c, n, false, d_acc);
+}
+
+int tpm_mlp_train_dev(tpm_ctx *ctx, const double *d_pos, size_t n, double a, size_t n_samples, int epochs,
+                      size_t batch, float lr, uint64_t seed, double *loss_history) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(d_pos && n > 0 && n <= c.max_n, "bad particle set");
+        TPM_REQUIRE((c.p.flags & TPM_FLAG_TREEPM) != 0, "training needs the TreePM split (targets come from the walk)");
+        TPM_REQUIRE(epochs >= 0 && batch >= 32 && lr > 0, "bad training parameters");
+        if (!c.mlp.params) tpm::mlp_init(c, seed);
+        if (!c.acc_sr) TPM_HIP(hipMalloc(&c.acc_sr, 3 * c.max_n * sizeof(float)));
+        const double fscale = learned_fscale(c, a, n);
+        const double k = 1.5 * c.p.omega_m * c.p.box_size * c.p.box_size * c.p.box_size / (4.0 * M_PI * a * (double)n);
+        pm_part(c, d_pos, n, a, c.acc);
+        TPM_HIP(hipMemsetAsync(c.acc_sr, 0, 3 * n * sizeof(float), c.stream));
+        tpm::build_tree(c, n);
+        tpm::walk_short_range(c, n, k, c.acc_sr);
+        const size_t ns = std::min(n_samples, n);
+        // Evenly spaced particles of the Morton order (every particle equally likely: density weighted).
+        std::vector<uint32_t> sel(ns);
+        const uint64_t off = seed % std::max<size_t>(1, n / ns);
+        for (size_t t = 0; t < ns; ++t) sel[t] = (uint32_t)std::min<uint64_t>(n - 1, t * (uint64_t)n / ns + off);
+        uint32_t *d_sel = nullptr, *d_perm = nullptr;
+        uint16_t *Xall = nullptr;
+        float *Tall = nullptr;
+        auto cleanup = [&]() {
+            (void)hipFree(d_sel);
+            (void)hipFree(d_perm);
+            (void)hipFree(Xall);
+            (void)hipFree(Tall);
+        };
+        try {
+            TPM_HIP(hipMalloc(&d_sel, ns * sizeof(uint32_t)));
+            TPM_HIP(hipMalloc(&d_perm, ns * sizeof(uint32_t)));
+            TPM_HIP(hipMalloc(&Xall, ns * 64 * sizeof(uint16_t)));
+            TPM_HIP(hipMalloc(&Tall, ns * 3 * sizeof(float)));
+            TPM_HIP(hipMemcpyAsync(d_sel, sel.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, c.stream));
+            tpm::mlp_features(c, d_sel, 0, ns, Xall, (float)(1.0 / fscale));
+            tpm::mlp_targets(c, d_sel, ns, c.acc_sr, (float)(1.0 / fscale), Tall);
+            const size_t rows = round32(batch);
+            tpm::mlp_prepare(c, rows);
+            std::vector<uint32_t> perm(ns);
+            uint64_t st = seed * 0x9E3779B97F4A7C15ULL + 7;
+            auto next = [&]() {
+                st ^= st << 13;
+                st ^= st >> 7;
+                st ^= st << 17;
+                return st;
+            };
+            const size_t steps_total = (size_t)epochs * ((ns + rows - 1) / rows);
+            size_t step = 0;
+            for (int e = 0; e < epochs; ++e) {
+                for (size_t t = 0; t < ns; ++t) perm[t] = (uint32_t)t;
+                for (size_t t = ns - 1; t > 0; --t) std::swap(perm[t], perm[next() % (t + 1)]);
+                TPM_HIP(hipMemcpyAsync(d_perm, perm.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, c.stream));
+                double last = 0.0;
+                for (size_t b0 = 0; b0 < ns; b0 += rows) {
+                    const size_t cnt = std::min(rows, ns - b0);
+                    tpm::mlp_gather_batch(c, Xall, Tall, d_perm, b0, ns, rows);
+                    const bool want = b0 + rows >= ns;
+                    last = tpm::mlp_forward_backward(c, rows, cnt, want);
+                    // Linear decay of the learning rate to a tenth over the run.
+                    const float f = 1.0f - 0.9f * (float)step / (float)std::max<size_t>(1, steps_total);
+                    tpm::mlp_adam_step(c, lr * f);
+                    ++step;
+                }
+                if (loss_history) loss_history[e] = last;
+            }
+            TPM_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+        c.mlp.trained = true;
+        return TPM_OK;
+    });
+}
+
+__global__ void k_compact(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pos, size_t n,
+                          uint32_t *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) out[pos[i]] = (uint32_t)i;
+}
+
+int tpm_accelerations_learned_dev(tpm_ctx *ctx, const double *d_pos, size_t n, double a, double tol, float *d_acc,
+                                  uint64_t *n_fallback) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(d_pos && d_acc && n > 0 && n <= c.max_n, "bad particle set");
+        TPM_REQUIRE((c.p.flags & TPM_FLAG_TREEPM) != 0, "the learned term replaces the TreePM short-range force");
+        TPM_REQUIRE(c.mlp.params != nullptr, "the MLP is not initialised");
+        TPM_REQUIRE(tol >= 0, "tol must be >= 0");
+        if (!c.members) TPM_HIP(hipMalloc(&c.members, c.max_n * sizeof(uint32_t)));
+        if (!c.reject) TPM_HIP(hipMalloc(&c.reject, c.max_n * sizeof(uint32_t)));
+        const double fscale = learned_fscale(c, a, n);
+        const double k = 1.5 * c.p.omega_m * c.p.box_size * c.p.box_size * c.p.box_size / (4.0 * M_PI * a * (double)n);
+        tpm::Timer tm(c);
+        tm.mark();
+        pm_part(c, d_pos, n, a, d_acc);
+        tm.mark();
+        const size_t chunk = 1u << 20;
+        tpm::mlp_prepare(c, chunk);
+        for (size_t first = 0; first < n; first += chunk) {
+            const size_t cnt = std::min(chunk, n - first);
+            tpm::mlp_features(c, nullptr, first, cnt, c.mlp.X, (float)(1.0 / fscale));
+            tpm::mlp_forward(c, round32(cnt), false);
+            tpm::mlp_apply(c, cnt, first, (float)fscale, (float)tol, d_acc, c.reject);
+        }
+        tm.mark();
+        // Members = rejected sorted indices (exclusive scan + scatter).
+        size_t bytes = c.gscan_bytes;
+        TPM_HIP(tpm::rocprim_exclusive_scan_u32(c.gscan_tmp, bytes, c.reject, c.gscratch, n, c.stream));
+        hipLaunchKernelGGL(k_compact, dim3(tpm::blocks(n, 256)), dim3(256), 0, c.stream, c.reject, c.gscratch, n,
+                           c.members);
+        uint32_t last[2];
+        TPM_HIP(hipMemcpyAsync(&last[0], c.gscratch + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipMemcpyAsync(&last[1], c.reject + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        const size_t nrej = (size_t)last[0] + last[1];
+        if (n_fallback) *n_fallback = nrej;
+        if (nrej > 0) {
+            tpm::build_tree(c, n);
+            tpm::walk_short_range(c, n, k, d_acc, c.members, nrej);
+        }
+        tm.mark();
+        if (c.timing) {
+            TPM_HIP(hipStreamSynchronize(c.stream));
+            c.stats.ms_fft = tm.ms(0, 1);
+            c.stats.ms_interp = tm.ms(1, 2);
+            c.stats.ms_walk = tm.ms(2, 3);
+            c.stats.ms_total = tm.ms(0, 3);
+        }
         return TPM_OK;
     });
 }

@@ -69,6 +69,25 @@ struct Tree {
     TreeView view() const { return TreeView{range, link, box, com}; }
 };
 
+// MLP layer sizes: 64 (33 features, zero padded) -> 128 -> 128 -> 128 -> 32 (4 used).
+constexpr int kMlpIn[4] = {64, 128, 128, 128};
+constexpr int kMlpOut[4] = {128, 128, 128, 32};
+
+// Learned short-range correction state (mlp_kernels.hip).
+struct Mlp {
+    float *params = nullptr, *grads = nullptr, *m1 = nullptr, *m2 = nullptr;  // FP32 master + Adam
+    uint16_t *wt16[4] = {nullptr, nullptr, nullptr, nullptr};  // BF16 Wt [out x in]
+    uint16_t *w16[4] = {nullptr, nullptr, nullptr, nullptr};   // BF16 W [in x out]
+    int step = 0;
+    bool trained = false;
+    size_t cap = 0;  // rows of the work buffers
+    uint16_t *X = nullptr, *Xt = nullptr, *Y[3] = {nullptr, nullptr, nullptr}, *Yt[3] = {nullptr, nullptr, nullptr};
+    float *Z[3] = {nullptr, nullptr, nullptr}, *C = nullptr, *Zout = nullptr, *dY = nullptr, *T = nullptr,
+          *Wrow = nullptr, *loss_part = nullptr;
+    uint16_t *dZ = nullptr, *dZt = nullptr;
+    std::vector<void *> work;
+};
+
 struct Ctx {
     tpm_params p{};
     size_t max_n = 0;
@@ -111,6 +130,10 @@ struct Ctx {
     size_t judge_cap = 0;
 
     Tree tree;
+    Mlp mlp;
+    float *acc_sr = nullptr;     // short-range force (caller order) while training
+    uint32_t *members = nullptr; // walk members (sorted indices) for partial walks
+    uint32_t *reject = nullptr;  // per sorted particle: 1 = learned estimate rejected
     float2 *stab = nullptr;     // S(u) table of the walk (kSTab + 1 entries)
     double stab_scale = 0.0;
     int num_cu = 0;
@@ -160,14 +183,16 @@ void build_tree(Ctx &c, size_t n);
 
 // Walk groups: runs of at most 64 sorted particles that never cross a cube of octree level
 // c.group_level (sets c.gstart, c.ngroups).
-void build_groups(Ctx &c, size_t n);
+void build_groups(Ctx &c, size_t n, const uint64_t *keys = nullptr);
 size_t group_scan_bytes(size_t max_n);
 
 // S(u) table of the group walk for the context's r_split and r_cut.
 void make_short_range_table(Ctx &c);
 
 // Short-range force of every particle added into acc[idx[i]] (group walk or reference walk).
-void walk_short_range(Ctx &c, size_t n, double k, float *d_acc);
+// With members != null only the `count` listed sorted particles are walked (all n are sources).
+void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *members = nullptr,
+                      size_t count = 0);
 // FP64 reference walk for listed targets (caller indices); writes 3 doubles per target.
 void walk_judge_f64(Ctx &c, size_t n, const uint32_t *d_targets_sorted, size_t m, double k, double theta,
                     double *d_out);
@@ -198,6 +223,22 @@ void drift(Ctx &c, double *d_pos, const double *d_mom, size_t n, double f);
 // background, positions written to d_pos (3 n doubles).
 void nfw_halos(Ctx &c, const tpm_nfw_spec &sp, size_t n, double *d_pos);
 using NfwSpec = tpm_nfw_spec;
+
+// MLP (mlp_kernels.hip).
+size_t mlp_param_count();
+void mlp_init(Ctx &c, uint64_t seed);
+void mlp_refresh_bf16(Ctx &c);
+void mlp_prepare(Ctx &c, size_t rows);
+void mlp_forward(Ctx &c, size_t rows, bool train);
+double mlp_forward_backward(Ctx &c, size_t rows, size_t count, bool want_loss);
+void mlp_pack_rows(Ctx &c, const float *d_x, const float *d_t, size_t count, size_t rows);
+size_t mlp_weight_offset(int l);
+void mlp_adam_step(Ctx &c, float lr);
+void mlp_features(Ctx &c, const uint32_t *d_sel, size_t first, size_t count, uint16_t *X, float inv_fscale);
+void mlp_targets(Ctx &c, const uint32_t *d_sel, size_t count, const float *acc_sr, float inv_fscale, float *T);
+void mlp_gather_batch(Ctx &c, const uint16_t *Xall, const float *Tall, const uint32_t *perm, size_t b0,
+                      size_t count, size_t rows);
+void mlp_apply(Ctx &c, size_t count, size_t first, float fscale, float tol, float *d_acc, uint32_t *reject);
 
 // Plans and tables (fft_kernels.hip).
 void make_plan(Ctx &c);

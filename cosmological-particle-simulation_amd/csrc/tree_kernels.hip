@@ -311,7 +311,8 @@ size_t group_scan_bytes(size_t max_n) {
     return std::max(a, b);
 }
 
-void build_groups(Ctx &c, size_t n) {
+void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
+    if (!keys) keys = c.keys;
     if (n == 0) {
         c.ngroups = 0;
         return;
@@ -324,7 +325,7 @@ void build_groups(Ctx &c, size_t n) {
     const int shift = 3 * (kKeyBits - level);
     uint32_t *segval = c.gscratch;
     uint32_t *tmp = c.gscratch + c.max_n;
-    hipLaunchKernelGGL(k_group_seg, dim3(blocks(n, 256)), dim3(256), 0, c.stream, c.keys, n, shift, segval);
+    hipLaunchKernelGGL(k_group_seg, dim3(blocks(n, 256)), dim3(256), 0, c.stream, keys, n, shift, segval);
     TPM_HIP(hipGetLastError());
     size_t bytes = c.gscan_bytes;
     TPM_HIP(rocprim::inclusive_scan(c.gscan_tmp, bytes, segval, tmp, n, rocprim::maximum<uint32_t>(), c.stream));
@@ -339,6 +340,11 @@ void build_groups(Ctx &c, size_t n) {
     TPM_HIP(hipMemcpyAsync(&last[1], segval + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
     TPM_HIP(hipStreamSynchronize(c.stream));
     c.ngroups = (size_t)last[0] + last[1];
+}
+
+hipError_t rocprim_exclusive_scan_u32(void *tmp, size_t &bytes, const uint32_t *in, uint32_t *out, size_t n,
+                                      hipStream_t s) {
+    return rocprim::exclusive_scan(tmp, bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), s);
 }
 
 size_t sort_temp_bytes(size_t max_n) {

@@ -37,6 +37,7 @@ struct WalkArgs {
     const double *spos;
     const uint32_t *idx;
     const uint32_t *gstart;
+    const uint32_t *members;  // sorted indices of the walked particles (null: all, in order)
     size_t n;
     TreeView t;
     double box;
@@ -448,9 +449,10 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
     const int lane = threadIdx.x & 63;
     const size_t g0 = w.gstart[g];
     const size_t g1 = w.gstart[g + 1];
-    const size_t i = g0 + lane;
-    const bool active = i < g1;
-    const size_t ii = active ? i : g0;
+    const bool active = g0 + lane < g1;
+    // Sorted index of this lane's particle (through the member list for a partial walk).
+    const size_t i = w.members ? (size_t)w.members[active ? g0 + lane : g0] : (active ? g0 + lane : g0);
+    const size_t ii = i;
     const double px = w.spos[3 * ii], py = w.spos[3 * ii + 1], pz = w.spos[3 * ii + 2];
     const double mnx = wave_min(px), mny = wave_min(py), mnz = wave_min(pz);
     const double mxx = wave_max(px), mxy = wave_max(py), mxz = wave_max(pz);
@@ -622,9 +624,17 @@ __global__ void __launch_bounds__(128) k_ref_walk(RefWalkArgs<T> w) {
 
 static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-void walk_short_range(Ctx &c, size_t n, double k, float *d_acc) {
+__global__ void k_gather_keys(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ members, size_t count,
+                              uint64_t *__restrict__ out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) out[t] = keys[members[t]];
+}
+
+void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *members, size_t count) {
     if (n == 0) return;
+    if (members && count == 0) return;
     if (c.p.flags & TPM_FLAG_REF_WALK) {
+        TPM_REQUIRE(members == nullptr, "the reference walk does not take a member list");
         RefWalkArgs<float> w{};
         w.spos = c.spos;
         w.idx = c.idx;
@@ -663,8 +673,17 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc) {
     w.stab = c.stab;
     w.acc = d_acc;
     w.counter = c.counter;
-    build_groups(c, n);
+    if (members) {
+        // Keys of the members, in member order, for the group segmentation.
+        hipLaunchKernelGGL(k_gather_keys, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c.stream, c.keys,
+                           members, count, c.keys_alt);
+        TPM_HIP(hipGetLastError());
+        build_groups(c, count, c.keys_alt);
+    } else {
+        build_groups(c, n);
+    }
     w.gstart = c.gstart;
+    w.members = members;
     w.ngroups = (uint32_t)c.ngroups;
     w.next_group = reinterpret_cast<uint32_t *>(c.flag) + 2;
     TPM_HIP(hipMemsetAsync(w.next_group, 0, sizeof(uint32_t), c.stream));

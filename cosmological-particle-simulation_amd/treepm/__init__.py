@@ -43,6 +43,8 @@ EXPORTS = [
     "tpm_short_range_f32", "tpm_short_range_f64", "tpm_judge_f64",
     "tpm_zeldovich", "tpm_zeldovich_dev", "tpm_kdk_factors", "tpm_step_to_dev", "tpm_run",
     "tpm_device_alloc", "tpm_device_free", "tpm_memcpy", "tpm_sync", "tpm_nfw_halos", "tpm_nfw_halos_dev",
+    "tpm_mlp_param_count", "tpm_mlp_init", "tpm_mlp_get_params", "tpm_mlp_set_params", "tpm_mlp_gradients",
+    "tpm_mlp_predict", "tpm_mlp_train_dev", "tpm_accelerations_learned_dev",
 ]
 
 
@@ -156,6 +158,14 @@ def load():
         "tpm_sync": (ctypes.c_int, [P]),
         "tpm_nfw_halos": (ctypes.c_int, [P, ctypes.POINTER(NfwSpec), S, P]),
         "tpm_nfw_halos_dev": (ctypes.c_int, [P, ctypes.POINTER(NfwSpec), S, P, P]),
+        "tpm_mlp_param_count": (S, []),
+        "tpm_mlp_init": (ctypes.c_int, [P, ctypes.c_uint64]),
+        "tpm_mlp_get_params": (ctypes.c_int, [P, P, S]),
+        "tpm_mlp_set_params": (ctypes.c_int, [P, P, S]),
+        "tpm_mlp_gradients": (ctypes.c_int, [P, P, P, S, P, ctypes.POINTER(ctypes.c_double)]),
+        "tpm_mlp_predict": (ctypes.c_int, [P, P, S, P]),
+        "tpm_mlp_train_dev": (ctypes.c_int, [P, P, S, D, S, ctypes.c_int, S, F, ctypes.c_uint64, P]),
+        "tpm_accelerations_learned_dev": (ctypes.c_int, [P, P, S, D, D, P, ctypes.POINTER(ctypes.c_uint64)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -351,6 +361,46 @@ class Solver:
     def nfw_halos_dev(self, spec, n, d_pos, stream=0):
         self._check(self.lib.tpm_nfw_halos_dev(self.h, ctypes.byref(spec), int(n), ctypes.c_void_p(d_pos),
                                                ctypes.c_void_p(stream or None)))
+
+    # ------------------------------------------------------------ learned correction
+    def mlp_init(self, seed=1):
+        self._check(self.lib.tpm_mlp_init(self.h, int(seed)))
+
+    def mlp_params(self):
+        n = self.lib.tpm_mlp_param_count()
+        out = np.empty(n, np.float32)
+        self._check(self.lib.tpm_mlp_get_params(self.h, _ptr(out), n))
+        return out
+
+    def mlp_set_params(self, params):
+        params = _f32(params).reshape(-1)
+        self._check(self.lib.tpm_mlp_set_params(self.h, _ptr(params), params.size))
+
+    def mlp_gradients(self, x, t):
+        x = _f32(x, (-1, 33))
+        t = _f32(t, (-1, 3))
+        g = np.empty(self.lib.tpm_mlp_param_count(), np.float32)
+        loss = ctypes.c_double()
+        self._check(self.lib.tpm_mlp_gradients(self.h, _ptr(x), _ptr(t), len(x), _ptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def mlp_predict(self, x):
+        x = _f32(x, (-1, 33))
+        out = np.empty((len(x), 4), np.float32)
+        self._check(self.lib.tpm_mlp_predict(self.h, _ptr(x), len(x), _ptr(out)))
+        return out
+
+    def mlp_train_dev(self, d_pos, n, a, n_samples, epochs, batch=8192, lr=1e-3, seed=1):
+        hist = np.zeros(max(int(epochs), 1), np.float64)
+        self._check(self.lib.tpm_mlp_train_dev(self.h, ctypes.c_void_p(d_pos), int(n), float(a), int(n_samples),
+                                               int(epochs), int(batch), float(lr), int(seed), _ptr(hist)))
+        return hist[:int(epochs)]
+
+    def accelerations_learned_dev(self, d_pos, n, a, tol, d_acc):
+        nf = ctypes.c_uint64()
+        self._check(self.lib.tpm_accelerations_learned_dev(self.h, ctypes.c_void_p(d_pos), int(n), float(a),
+                                                           float(tol), ctypes.c_void_p(d_acc), ctypes.byref(nf)))
+        return int(nf.value)
 
     def sync(self):
         self._check(self.lib.tpm_sync(self.h))

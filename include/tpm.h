@@ -183,6 +183,40 @@ typedef struct tpm_nfw_spec {
 int tpm_nfw_halos(tpm_ctx *ctx, const tpm_nfw_spec *spec, size_t n, double *pos);
 int tpm_nfw_halos_dev(tpm_ctx *ctx, const tpm_nfw_spec *spec, size_t n, double *d_pos, void *stream);
 
+/*
+ * Learned short-range correction (no reference counterpart; the exact walk is its fallback and
+ * judge). A 64 -> 128 -> 128 -> 128 -> 4 SiLU MLP with BF16 weights and activations on MFMA (FP32
+ * accumulation) predicts the short-range force from 33 features per particle (log(1 + delta) on
+ * the 3x3x3 cells around it, the PM force, its offset from the cell centre in cells) and a
+ * log-variance used as the error estimate. Forces are in the compressed units slog(F / (K/h^2)),
+ * slog(x) = sign(x) log(1 + |x|), with K the coupling and h the cell size.
+ * Parameters are FP32 masters (tpm_mlp_param_count() floats, layout: per layer Wt [out x in] then
+ * bias, with in = 64, 128, 128, 128 and out = 128, 128, 128, 32; only 33 inputs and 4 outputs used).
+ */
+size_t tpm_mlp_param_count(void);
+int tpm_mlp_init(tpm_ctx *ctx, uint64_t seed);
+int tpm_mlp_get_params(tpm_ctx *ctx, float *params, size_t count);
+int tpm_mlp_set_params(tpm_ctx *ctx, const float *params, size_t count);
+/* Forward + backward (no update) on host samples: x rows x 33, t rows x 3 (compressed units);
+ * grads: param_count. */
+int tpm_mlp_gradients(tpm_ctx *ctx, const float *x, const float *t, size_t rows, float *grads, double *loss);
+/* Forward on host samples: x rows x 33 -> out rows x 4 (force in K/h^2, log variance). */
+int tpm_mlp_predict(tpm_ctx *ctx, const float *x, size_t rows, float *out);
+/*
+ * Trains on n_samples particles of d_pos (device, FP64), with the walk's short-range force as the
+ * target, for `epochs` passes of Adam (batch rows, learning rate lr). loss_history (epochs
+ * entries, may be NULL) receives each epoch's last-batch loss.
+ */
+int tpm_mlp_train_dev(tpm_ctx *ctx, const double *d_pos, size_t n, double a, size_t n_samples, int epochs,
+                      size_t batch, float lr, uint64_t seed, double *loss_history);
+/*
+ * TreePM force with the learned short-range term: PM force + the MLP's prediction wherever its
+ * estimated RMS error is within tol * |a|; every other particle is walked exactly. n_fallback
+ * (may be NULL) receives the number of walked particles.
+ */
+int tpm_accelerations_learned_dev(tpm_ctx *ctx, const double *d_pos, size_t n, double a, double tol, float *d_acc,
+                                  uint64_t *n_fallback);
+
 /* Background cosmology for the KDK step factors (src/cosmology.rs:9-34, flat: Or + Om + OL = 1). */
 typedef struct tpm_cosmology {
     double omega_r;
