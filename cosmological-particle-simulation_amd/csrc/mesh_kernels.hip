@@ -8,6 +8,7 @@
 // the remainder so each particle adds exactly round(mass * 2^32), and the quanta are summed
 // with 64-bit integer atomics. Integer addition commutes, so the grid is bit-exact from run to
 // run whatever the order of the atomics, and the total mass is exact.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 
@@ -241,6 +242,11 @@ __global__ void k_interp_judge(const double *__restrict__ pos, const uint32_t *_
     }
 }
 
+__global__ void k_zero32(uint32_t *__restrict__ p, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 __global__ void k_f32_to_f64(const float *__restrict__ src, double *__restrict__ dst, size_t count) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) dst[i] = (double)src[i];
@@ -254,7 +260,7 @@ __global__ void k_check_finite(const float *__restrict__ v, size_t count, int *f
 static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double *pos_src, bool sorted) {
-    TPM_HIP(hipMemsetAsync(c.grid, 0, c.m * sizeof(unsigned long long), c.stream));
+    zero_fill(c, c.grid, c.m * sizeof(unsigned long long));
     if (n == 0) return;
     if (sorted) {
         const unsigned nb = (unsigned)((n + kDepChunk - 1) / kDepChunk);
@@ -312,6 +318,14 @@ void interp_judge_f64(Ctx &c, const uint32_t *d_targets_sorted, size_t m, double
     TPM_HIP(hipGetLastError());
 }
 
+void zero_fill(Ctx &c, void *ptr, size_t bytes) {
+    const size_t words = bytes / 4;
+    if (words == 0) return;
+    const unsigned grid = (unsigned)std::min<size_t>(blocks(words, 256), 8192);
+    hipLaunchKernelGGL(k_zero32, dim3(grid), dim3(256), 0, c.stream, (uint32_t *)ptr, words);
+    TPM_HIP(hipGetLastError());
+}
+
 void convert_f32_to_f64(Ctx &c, const float *src, double *dst, size_t count) {
     if (count == 0) return;
     hipLaunchKernelGGL(k_f32_to_f64, dim3(blocks(count, 256)), dim3(256), 0, c.stream, src, dst, count);
@@ -319,7 +333,7 @@ void convert_f32_to_f64(Ctx &c, const float *src, double *dst, size_t count) {
 }
 
 void check_finite(Ctx &c, const float *d_acc, size_t count) {
-    TPM_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+    zero_fill(c, c.flag, sizeof(int));
     if (count == 0) return;
     hipLaunchKernelGGL(k_check_finite, dim3(blocks(count, 256)), dim3(256), 0, c.stream, d_acc, count, c.flag);
     TPM_HIP(hipGetLastError());

@@ -58,8 +58,8 @@ static void validate(const tpm_params &p) {
 static void free_all(Ctx &c) {
     void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.sort_tmp, c.spos,
                     c.counter, c.flag, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
-                    c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.scan_tmp,
-                    c.scan_tmp_storage, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp, c.stab,
+                    c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.tree_scan,
+                    c.ngroups_dev, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp, c.stab,
                     c.acc_sr, c.members, c.reject, c.mlp.params, c.mlp.grads, c.mlp.m1, c.mlp.m2};
     for (int l = 0; l < 4; ++l) {
         if (c.mlp.wt16[l]) (void)hipFree(c.mlp.wt16[l]);
@@ -99,6 +99,7 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     TPM_HIP(hipMalloc(&c.gscratch, 2 * mn * sizeof(uint32_t)));
     c.gscan_bytes = group_scan_bytes(mn);
     TPM_HIP(hipMalloc(&c.gscan_tmp, c.gscan_bytes ? c.gscan_bytes : 16));
+    TPM_HIP(hipMalloc(&c.ngroups_dev, 16));
     TPM_HIP(hipMalloc(&c.counter, 8 * sizeof(unsigned long long)));
     TPM_HIP(hipMalloc(&c.flag, sizeof(int) * 4));
     TPM_HIP(hipMalloc(&c.grid, c.m * sizeof(unsigned long long)));
@@ -149,13 +150,25 @@ static double coupling(const Ctx &c, double a, size_t n, bool f32) {
     return 1.5 * c.p.omega_m * l * l * l / (4.0 * M_PI * a * (double)n);
 }
 
+// Node and group counts of the last build (a synchronous read; statistics only).
+static void read_tree_stats(Ctx &c, bool built) {
+    uint32_t info[2] = {0, 0}, ng = 0;
+    if (built && c.tree_info) {
+        TPM_HIP(hipMemcpy(info, c.tree_info, sizeof(info), hipMemcpyDeviceToHost));
+        TPM_HIP(hipMemcpy(&ng, c.ngroups_dev, sizeof(ng), hipMemcpyDeviceToHost));
+        if (info[1]) throw Error{TPM_E_CAPACITY, "octree node capacity exceeded"};
+    }
+    c.stats.nodes = info[0];
+    c.stats.groups = ng;
+}
+
 // The force evaluation on device-resident positions (caller order, FP64).
 static void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float *d_acc, float *d_delta) {
     TPM_REQUIRE(n <= c.max_n, "more particles than the context capacity");
     TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
     const bool treepm = (c.p.flags & TPM_FLAG_TREEPM) != 0;
     Timer tm(c);
-    TPM_HIP(hipMemsetAsync(c.counter, 0, 8 * sizeof(unsigned long long), c.stream));
+    zero_fill(c, c.counter, 8 * sizeof(unsigned long long));
     tm.mark();
     sort_particles(c, d_pos, n);
     tm.mark();
@@ -164,7 +177,7 @@ static void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w,
     if (n > 0) {
         pm_from_grid(c, n, a, treepm ? c.p.r_split : 0.0, d_delta);
     } else {
-        if (d_delta) TPM_HIP(hipMemsetAsync(d_delta, 0, c.m * sizeof(float), c.stream));
+        if (d_delta) zero_fill(c, d_delta, c.m * sizeof(float));
     }
     tm.mark();
     interp_force(c, n, f32w, d_acc);
@@ -191,8 +204,7 @@ static void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w,
         unsigned long long cnt[4] = {0, 0, 0, 0};
         TPM_HIP(hipMemcpy(cnt, c.counter, sizeof(cnt), hipMemcpyDeviceToHost));
         c.stats.interactions = cnt[0];
-        c.stats.nodes = c.tree.count;
-        c.stats.groups = c.ngroups;
+        read_tree_stats(c, treepm && n > 0);
         c.stats.listed = cnt[2];
         c.stats.kept = cnt[3];
         if (cnt[1]) throw Error{TPM_E_DEVICE, "walk stack overflow"};
@@ -201,9 +213,13 @@ static void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w,
 
 static void finish_checks(Ctx &c) {
     unsigned long long ovf = 0;
+    uint32_t info[2] = {0, 0};
     TPM_HIP(hipMemcpyAsync(&ovf, c.counter + 1, sizeof(ovf), hipMemcpyDeviceToHost, c.stream));
+    if (c.tree_info)
+        TPM_HIP(hipMemcpyAsync(info, c.tree_info, sizeof(info), hipMemcpyDeviceToHost, c.stream));
     TPM_HIP(hipStreamSynchronize(c.stream));
     if (ovf) throw Error{TPM_E_DEVICE, "walk stack overflow"};
+    if (info[1]) throw Error{TPM_E_CAPACITY, "octree node capacity exceeded"};
     if (c.p.flags & TPM_FLAG_CHECK_FINITE) {
         int bad = 0;
         TPM_HIP(hipMemcpyAsync(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -418,9 +434,9 @@ static int short_range(tpm_ctx *ctx, const void *pos, size_t n, double k, float 
         TPM_REQUIRE(c.p.r_split > 0 && c.p.r_cut > 0, "short-range force needs r_split > 0 and r_cut > 0");
         tpm::upload_positions(c, pos, n, f32);
         if (n == 0) return TPM_OK;
-        TPM_HIP(hipMemsetAsync(c.counter, 0, 8 * sizeof(unsigned long long), c.stream));
+        tpm::zero_fill(c, c.counter, 8 * sizeof(unsigned long long));
         tpm::sort_particles(c, c.pos, n);
-        TPM_HIP(hipMemsetAsync(c.acc, 0, 3 * n * sizeof(float), c.stream));
+        tpm::zero_fill(c, c.acc, 3 * n * sizeof(float));
         tpm::build_tree(c, n);
         tpm::walk_short_range(c, n, k, c.acc);
         TPM_HIP(hipMemcpyAsync(acc, c.acc, 3 * n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
@@ -428,7 +444,7 @@ static int short_range(tpm_ctx *ctx, const void *pos, size_t n, double k, float 
         TPM_HIP(hipMemcpyAsync(&cnt, c.counter, sizeof(cnt), hipMemcpyDeviceToHost, c.stream));
         TPM_HIP(hipStreamSynchronize(c.stream));
         c.stats.interactions = cnt;
-        c.stats.nodes = c.tree.count;
+        tpm::read_tree_stats(c, true);
         return TPM_OK;
     });
 }
@@ -464,7 +480,7 @@ int tpm_judge_f64(tpm_ctx *ctx, const double *pos, size_t n, const uint32_t *idx
         double *d_sr = d_pm + 3 * m;
         TPM_HIP(hipMemcpyAsync(d_caller, idx, m * sizeof(uint32_t), hipMemcpyHostToDevice, c.stream));
         hipLaunchKernelGGL(tpm::k_inverse_perm, dim3(tpm::blocks(n, 256)), dim3(256), 0, c.stream, c.idx, n, c.inv);
-        TPM_HIP(hipMemsetAsync(c.flag, 0, sizeof(int), c.stream));
+        tpm::zero_fill(c, c.flag, sizeof(int));
         hipLaunchKernelGGL(tpm::k_map_targets, dim3(tpm::blocks(m, 256)), dim3(256), 0, c.stream, c.inv, d_caller, m, n,
                            d_sorted, c.flag);
         TPM_HIP(hipGetLastError());
@@ -624,7 +640,7 @@ int tpm_mlp_train_dev(tpm_ctx *ctx, const double *d_pos, size_t n, double a, siz
         const double fscale = learned_fscale(c, a, n);
         const double k = 1.5 * c.p.omega_m * c.p.box_size * c.p.box_size * c.p.box_size / (4.0 * M_PI * a * (double)n);
         pm_part(c, d_pos, n, a, c.acc);
-        TPM_HIP(hipMemsetAsync(c.acc_sr, 0, 3 * n * sizeof(float), c.stream));
+        tpm::zero_fill(c, c.acc_sr, 3 * n * sizeof(float));
         tpm::build_tree(c, n);
         tpm::walk_short_range(c, n, k, c.acc_sr);
         const size_t ns = std::min(n_samples, n);

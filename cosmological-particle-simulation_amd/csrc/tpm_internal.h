@@ -65,7 +65,6 @@ struct Tree {
     uint4 *link = nullptr;
     float4 *box = nullptr;
     double4 *com = nullptr;
-    std::vector<size_t> level_begin;  // node index where each level starts (+ end sentinel)
     TreeView view() const { return TreeView{range, link, box, com}; }
 };
 
@@ -143,12 +142,11 @@ struct Ctx {
     uint32_t *gscratch = nullptr;   // 2 max_n words
     void *gscan_tmp = nullptr;
     size_t gscan_bytes = 0;
-    size_t ngroups = 0;
     int group_level = 2;
-    uint32_t *scan_tmp = nullptr;  // per-level child counts / offsets
-    size_t scan_cap = 0;
-    void *scan_tmp_storage = nullptr;
-    size_t scan_tmp_bytes = 0;
+    int group_level_shift = 0;  // autotuned offset of the group cube level
+    uint32_t *tree_scan = nullptr;  // per-level child counts, offsets, scan partials, level table
+    uint32_t *tree_info = nullptr;  // [0] node count, [1] capacity exceeded
+    uint32_t *ngroups_dev = nullptr;  // walk group count (device)
 };
 
 // ---------------------------------------------------------------------------------------
@@ -201,6 +199,9 @@ void walk_judge_f64(Ctx &c, size_t n, const uint32_t *d_targets_sorted, size_t m
 void interp_judge_f64(Ctx &c, const uint32_t *d_targets_sorted, size_t m, double *d_out);
 
 // Misc.
+// Zero `bytes` (a multiple of 4) at ptr with a fill kernel (graph-capture safe, unlike memset).
+void zero_fill(Ctx &c, void *ptr, size_t bytes);
+size_t tree_capacity(size_t n);
 void convert_f32_to_f64(Ctx &c, const float *src, double *dst, size_t count);
 void check_finite(Ctx &c, const float *d_acc, size_t count);
 

@@ -76,92 +76,201 @@ __device__ __forceinline__ uint32_t lower_digit(const uint64_t *keys, uint32_t l
     return lo;
 }
 
+// The octree is built level by level entirely on the device (no host synchronisation, so the
+// build can be captured in a HIP graph): lv[l] is the first node of level l, lv[l + 1] its end;
+// each level's child counts are scanned by a fixed-grid three-phase scan and the next level's
+// range is written to lv[l + 2].
+constexpr int kScanBlocks = 256;
+
 // Pass 1 over one level: decide which nodes split and count their non-empty octants.
-__global__ void k_count_children(const uint64_t *__restrict__ keys, TreeView t, size_t lb, size_t le, int level,
-                                 uint32_t ref_leaf_rule, uint32_t *__restrict__ nchild) {
-    const size_t q = lb + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= le) return;
-    const uint2 r = t.range[q];
-    const uint32_t cnt = r.y - r.x;
-    uint32_t mask = 0, nc = 0;
-    if (cnt > (uint32_t)kLeafSize && level < kMaxDepth) {
+__global__ void __launch_bounds__(256) k_count_children(const uint64_t *__restrict__ keys, TreeView t,
+                                                        const uint32_t *__restrict__ lv, int level,
+                                                        uint32_t ref_leaf_rule, uint32_t *__restrict__ nchild) {
+    const uint32_t lb = lv[level], le = lv[level + 1];
+    for (uint32_t q = lb + blockIdx.x * blockDim.x + threadIdx.x; q < le; q += gridDim.x * blockDim.x) {
+        const uint2 r = t.range[q];
+        const uint32_t cnt = r.y - r.x;
+        uint32_t mask = 0, nc = 0;
+        if (cnt > (uint32_t)kLeafSize && level < kMaxDepth) {
+            uint32_t b[9];
+            b[0] = r.x;
+            b[8] = r.y;
+            for (int o = 1; o < 8; ++o) b[o] = lower_digit(keys, b[o - 1], r.y, level + 1, o);
+            for (int o = 0; o < 8; ++o)
+                if (b[o + 1] > b[o]) {
+                    mask |= 1u << o;
+                    ++nc;
+                }
+        }
+        // Walk-leaf: no children, or (reference rule) no child in octant 0 (src/tree.rs:280).
+        const uint32_t walk_leaf = (nc == 0 || (ref_leaf_rule && !(mask & 1u))) ? 1u : 0u;
+        uint4 l = t.link[q];
+        l.y = nc;
+        l.z = mask;
+        l.w = (uint32_t)level | (walk_leaf << 8);
+        t.link[q] = l;
+        nchild[q - lb] = nc;
+    }
+}
+
+// Chunk [c0, c1) of a scan over cnt values handled by block b of kScanBlocks.
+__device__ inline void scan_chunk(uint32_t cnt, uint32_t b, uint32_t &c0, uint32_t &c1) {
+    const uint32_t per = (cnt + kScanBlocks - 1) / kScanBlocks;
+    c0 = min(cnt, b * per);
+    c1 = min(cnt, c0 + per);
+}
+
+__global__ void __launch_bounds__(256) k_scan_partials(const uint32_t *__restrict__ in, const uint32_t *__restrict__ lv,
+                                                       int level, uint32_t *__restrict__ part) {
+    __shared__ uint32_t sh[256];
+    const uint32_t cnt = lv[level + 1] - lv[level];
+    uint32_t c0, c1;
+    scan_chunk(cnt, blockIdx.x, c0, c1);
+    uint32_t s = 0;
+    for (uint32_t i = c0 + threadIdx.x; i < c1; i += 256) s += in[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Exclusive scan of the kScanBlocks partials (one block); the total gives the next level's end.
+__global__ void __launch_bounds__(kScanBlocks) k_scan_top(uint32_t *__restrict__ part, uint32_t *__restrict__ lv,
+                                                          int level, uint32_t cap, uint32_t *__restrict__ info) {
+    __shared__ uint32_t sh[kScanBlocks];
+    const uint32_t v = part[threadIdx.x];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < kScanBlocks; d <<= 1) {
+        const uint32_t add = (int)threadIdx.x >= d ? sh[threadIdx.x - d] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    part[threadIdx.x] = sh[threadIdx.x] - v;
+    if (threadIdx.x == kScanBlocks - 1) {
+        const uint32_t le = lv[level + 1];
+        uint64_t end = (uint64_t)le + sh[threadIdx.x];
+        if (end > cap) {
+            info[1] = 1u;  // capacity exceeded: children past the capacity are not created
+            end = cap;
+        }
+        lv[level + 2] = (uint32_t)end;
+        info[0] = (uint32_t)end;
+    }
+}
+
+// Exclusive scan within each block's chunk, offset by the block's partial.
+__global__ void __launch_bounds__(256) k_scan_finish(const uint32_t *__restrict__ in, const uint32_t *__restrict__ lv,
+                                                     int level, const uint32_t *__restrict__ part,
+                                                     uint32_t *__restrict__ out) {
+    __shared__ uint32_t sh[256];
+    const uint32_t cnt = lv[level + 1] - lv[level];
+    uint32_t c0, c1;
+    scan_chunk(cnt, blockIdx.x, c0, c1);
+    uint32_t carry = part[blockIdx.x];
+    for (uint32_t t0 = c0; t0 < c1; t0 += 256) {
+        const uint32_t i = t0 + threadIdx.x;
+        const uint32_t v = i < c1 ? in[i] : 0u;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 1; d < 256; d <<= 1) {
+            const uint32_t add = (int)threadIdx.x >= d ? sh[threadIdx.x - d] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < c1) out[i] = carry + sh[threadIdx.x] - v;
+        carry += sh[255];
+        __syncthreads();
+    }
+}
+
+// Pass 2: create the children of each split node at first = le + offset[q - lb].
+__global__ void __launch_bounds__(256) k_make_children(const uint64_t *__restrict__ keys, TreeView t,
+                                                       const uint32_t *__restrict__ lv, int level,
+                                                       const uint32_t *__restrict__ offset, uint32_t cap) {
+    const uint32_t lb = lv[level], le = lv[level + 1];
+    for (uint32_t q = lb + blockIdx.x * blockDim.x + threadIdx.x; q < le; q += gridDim.x * blockDim.x) {
+        uint4 l = t.link[q];
+        if (l.y == 0) {
+            l.x = 0xFFFFFFFFu;
+            t.link[q] = l;
+            continue;
+        }
+        const uint2 r = t.range[q];
         uint32_t b[9];
         b[0] = r.x;
         b[8] = r.y;
         for (int o = 1; o < 8; ++o) b[o] = lower_digit(keys, b[o - 1], r.y, level + 1, o);
-        for (int o = 0; o < 8; ++o)
-            if (b[o + 1] > b[o]) {
-                mask |= 1u << o;
-                ++nc;
-            }
-    }
-    // Walk-leaf: no children, or (reference rule) no child in octant 0 (src/tree.rs:280).
-    const uint32_t walk_leaf = (nc == 0 || (ref_leaf_rule && !(mask & 1u))) ? 1u : 0u;
-    uint4 l = t.link[q];
-    l.y = nc;
-    l.z = mask;
-    l.w = (uint32_t)level | (walk_leaf << 8);
-    t.link[q] = l;
-    nchild[q - lb] = nc;
-}
-
-// Pass 2: create the children of each split node at first = le + offset[q].
-__global__ void k_make_children(const uint64_t *__restrict__ keys, TreeView t, size_t lb, size_t le, int level,
-                                const uint32_t *__restrict__ offset) {
-    const size_t q = lb + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= le) return;
-    uint4 l = t.link[q];
-    if (l.y == 0) {
-        l.x = 0xFFFFFFFFu;
+        const uint64_t first = (uint64_t)le + offset[q - lb];
+        if (first + l.y > cap) {
+            // Over capacity (reported through info[1]): the node stays a leaf so the walk stays exact.
+            l.x = 0xFFFFFFFFu;
+            l.y = 0;
+            l.w |= 1u << 8;
+            t.link[q] = l;
+            continue;
+        }
+        l.x = (uint32_t)first;
         t.link[q] = l;
-        return;
-    }
-    const uint2 r = t.range[q];
-    uint32_t b[9];
-    b[0] = r.x;
-    b[8] = r.y;
-    for (int o = 1; o < 8; ++o) b[o] = lower_digit(keys, b[o - 1], r.y, level + 1, o);
-    const uint32_t first = (uint32_t)(le + offset[q - lb]);
-    l.x = first;
-    t.link[q] = l;
-    const float4 pb = t.box[q];
-    const float quarter = pb.w / 2.0f;
-    uint32_t ci = first;
-    for (int o = 0; o < 8; ++o) {
-        if (b[o + 1] == b[o]) continue;
-        t.range[ci] = make_uint2(b[o], b[o + 1]);
-        // src/tree.rs:243-245: centre[k] +- quarter in FP32.
-        t.box[ci] = make_float4(pb.x + ((o & 1) ? quarter : -quarter), pb.y + ((o & 2) ? quarter : -quarter),
-                                pb.z + ((o & 4) ? quarter : -quarter), quarter);
-        t.link[ci] = make_uint4(0xFFFFFFFFu, 0, 0, (uint32_t)(level + 1));
-        ++ci;
+        const float4 pb = t.box[q];
+        const float quarter = pb.w / 2.0f;
+        uint32_t ci = (uint32_t)first;
+        for (int o = 0; o < 8; ++o) {
+            if (b[o + 1] == b[o]) continue;
+            t.range[ci] = make_uint2(b[o], b[o + 1]);
+            // src/tree.rs:243-245: centre[k] +- quarter in FP32.
+            t.box[ci] = make_float4(pb.x + ((o & 1) ? quarter : -quarter), pb.y + ((o & 2) ? quarter : -quarter),
+                                    pb.z + ((o & 4) ? quarter : -quarter), quarter);
+            t.link[ci] = make_uint4(0xFFFFFFFFu, 0, 0, (uint32_t)(level + 1));
+            ++ci;
+        }
     }
 }
 
 // Centres of mass of one level, deepest level first: leaves from their particles, split nodes
 // from their children (count-weighted), all in FP64.
-__global__ void k_com_level(const double *__restrict__ spos, TreeView t, size_t lb, size_t le) {
-    const size_t q = lb + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= le) return;
-    const uint4 l = t.link[q];
-    const uint2 r = t.range[q];
-    double sx = 0, sy = 0, sz = 0;
-    if (l.y == 0) {
-        for (uint32_t j = r.x; j < r.y; ++j) {
-            sx += spos[3 * (size_t)j];
-            sy += spos[3 * (size_t)j + 1];
-            sz += spos[3 * (size_t)j + 2];
+__global__ void __launch_bounds__(256) k_com_level(const double *__restrict__ spos, TreeView t,
+                                                   const uint32_t *__restrict__ lv, int level) {
+    const uint32_t lb = lv[level], le = lv[level + 1];
+    for (uint32_t q = lb + blockIdx.x * blockDim.x + threadIdx.x; q < le; q += gridDim.x * blockDim.x) {
+        const uint4 l = t.link[q];
+        const uint2 r = t.range[q];
+        double sx = 0, sy = 0, sz = 0;
+        if (l.y == 0) {
+            for (uint32_t j = r.x; j < r.y; ++j) {
+                sx += spos[3 * (size_t)j];
+                sy += spos[3 * (size_t)j + 1];
+                sz += spos[3 * (size_t)j + 2];
+            }
+        } else {
+            for (uint32_t cix = 0; cix < l.y; ++cix) {
+                const double4 cc = t.com[l.x + cix];
+                sx += cc.x * cc.w;
+                sy += cc.y * cc.w;
+                sz += cc.z * cc.w;
+            }
         }
-    } else {
-        for (uint32_t cix = 0; cix < l.y; ++cix) {
-            const double4 cc = t.com[l.x + cix];
-            sx += cc.x * cc.w;
-            sy += cc.y * cc.w;
-            sz += cc.z * cc.w;
-        }
+        const double cnt = (double)(r.y - r.x);
+        t.com[q] = make_double4(sx / cnt, sy / cnt, sz / cnt, cnt);
     }
-    const double cnt = (double)(r.y - r.x);
-    t.com[q] = make_double4(sx / cnt, sy / cnt, sz / cnt, cnt);
+}
+
+__global__ void k_tree_root(TreeView t, uint32_t n, float half, uint32_t *__restrict__ lv,
+                            uint32_t *__restrict__ info) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    t.range[0] = make_uint2(0, n);
+    t.box[0] = make_float4(half, half, half, half);
+    t.link[0] = make_uint4(0xFFFFFFFFu, 0, 0, 0);
+    lv[0] = 0;
+    lv[1] = 1;
+    for (int l = 2; l < kMaxDepth + 3; ++l) lv[l] = 1;
+    info[0] = 1;
+    info[1] = 0;
 }
 
 static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
@@ -180,101 +289,56 @@ void sort_particles(Ctx &c, const double *d_pos, size_t n) {
     TPM_HIP(hipGetLastError());
 }
 
-static void grow_tree(Ctx &c, size_t need) {
+// Tree arrays for up to `cap` nodes (allocated once per capacity; never resized during a build).
+static void ensure_tree(Ctx &c, size_t cap) {
     Tree &t = c.tree;
-    if (need <= t.cap) return;
-    size_t cap = t.cap ? t.cap : 1024;
-    while (cap < need) cap *= 2;
-    uint2 *range;
-    uint4 *link;
-    float4 *box;
-    double4 *com;
-    TPM_HIP(hipMalloc(&range, cap * sizeof(uint2)));
-    TPM_HIP(hipMalloc(&link, cap * sizeof(uint4)));
-    TPM_HIP(hipMalloc(&box, cap * sizeof(float4)));
-    TPM_HIP(hipMalloc(&com, cap * sizeof(double4)));
-    if (t.count) {
-        TPM_HIP(hipMemcpyAsync(range, t.range, t.count * sizeof(uint2), hipMemcpyDeviceToDevice, c.stream));
-        TPM_HIP(hipMemcpyAsync(link, t.link, t.count * sizeof(uint4), hipMemcpyDeviceToDevice, c.stream));
-        TPM_HIP(hipMemcpyAsync(box, t.box, t.count * sizeof(float4), hipMemcpyDeviceToDevice, c.stream));
-        TPM_HIP(hipStreamSynchronize(c.stream));
-    }
+    if (cap <= t.cap) return;
     (void)hipFree(t.range);
     (void)hipFree(t.link);
     (void)hipFree(t.box);
     (void)hipFree(t.com);
-    t.range = range;
-    t.link = link;
-    t.box = box;
-    t.com = com;
+    (void)hipFree(c.tree_scan);
+    TPM_HIP(hipMalloc(&t.range, cap * sizeof(uint2)));
+    TPM_HIP(hipMalloc(&t.link, cap * sizeof(uint4)));
+    TPM_HIP(hipMalloc(&t.box, cap * sizeof(float4)));
+    TPM_HIP(hipMalloc(&t.com, cap * sizeof(double4)));
+    // child counts and offsets (cap each), scan partials, level table and info words
+    TPM_HIP(hipMalloc(&c.tree_scan, (2 * cap + kScanBlocks + kMaxDepth + 8) * sizeof(uint32_t)));
     t.cap = cap;
 }
 
-static void grow_scan(Ctx &c, size_t need) {
-    if (need <= c.scan_cap) return;
-    size_t cap = c.scan_cap ? c.scan_cap : 4096;
-    while (cap < need) cap *= 2;
-    (void)hipFree(c.scan_tmp);
-    TPM_HIP(hipMalloc(&c.scan_tmp, 2 * cap * sizeof(uint32_t) + 64));
-    c.scan_cap = cap;
-    size_t bytes = 0;
-    TPM_HIP(rocprim::exclusive_scan(nullptr, bytes, c.scan_tmp, c.scan_tmp + cap, 0u, cap, rocprim::plus<uint32_t>(),
-                                    c.stream));
-    if (bytes > c.scan_tmp_bytes) {
-        (void)hipFree(c.scan_tmp_storage);
-        TPM_HIP(hipMalloc(&c.scan_tmp_storage, bytes));
-        c.scan_tmp_bytes = bytes;
-    }
-}
+size_t tree_capacity(size_t n) { return std::max<size_t>(1024, 2 * n + 1024); }
 
 void build_tree(Ctx &c, size_t n) {
     Tree &t = c.tree;
-    t.level_begin.clear();
-    t.count = 0;
-    grow_tree(c, std::max<size_t>(1024, n / 2 + 64));
+    ensure_tree(c, tree_capacity(n));
+    uint32_t *nch = c.tree_scan;
+    uint32_t *off = nch + t.cap;
+    uint32_t *part = off + t.cap;
+    uint32_t *lv = part + kScanBlocks;
+    uint32_t *info = lv + kMaxDepth + 3;
+    c.tree_info = info;
     // Root: the whole box, centre L/2 and half L/2 in FP32 (src/tree.rs:180-181).
     const float half = (float)c.p.box_size / 2.0f;
-    const uint2 rr = make_uint2(0, (uint32_t)n);
-    const float4 rb = make_float4(half, half, half, half);
-    const uint4 rl = make_uint4(0xFFFFFFFFu, 0, 0, 0);
-    TPM_HIP(hipMemcpyAsync(t.range, &rr, sizeof(rr), hipMemcpyHostToDevice, c.stream));
-    TPM_HIP(hipMemcpyAsync(t.box, &rb, sizeof(rb), hipMemcpyHostToDevice, c.stream));
-    TPM_HIP(hipMemcpyAsync(t.link, &rl, sizeof(rl), hipMemcpyHostToDevice, c.stream));
-    t.count = 1;
-    t.level_begin.push_back(0);
+    hipLaunchKernelGGL(k_tree_root, dim3(1), dim3(64), 0, c.stream, t.view(), (uint32_t)n, half, lv, info);
+    TPM_HIP(hipGetLastError());
     const uint32_t ref_rule = (c.p.flags & TPM_FLAG_REF_LEAF_RULE) ? 1u : 0u;
-    size_t lb = 0, le = 1;
-    for (int level = 0; level <= kMaxDepth && le > lb; ++level) {
-        const size_t cnt = le - lb;
-        grow_scan(c, cnt);
-        uint32_t *nch = c.scan_tmp;
-        uint32_t *off = c.scan_tmp + c.scan_cap;
-        hipLaunchKernelGGL(k_count_children, dim3(blocks(cnt, 256)), dim3(256), 0, c.stream, c.keys, t.view(), lb, le, level,
-                           ref_rule, nch);
+    // Node-parallel kernels stride over each level with a grid sized for n particles.
+    const unsigned grid = std::max(1u, std::min(blocks(n, 256), 4096u));
+    for (int level = 0; level <= kMaxDepth; ++level) {
+        hipLaunchKernelGGL(k_count_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level, ref_rule,
+                           nch);
+        hipLaunchKernelGGL(k_scan_partials, dim3(kScanBlocks), dim3(256), 0, c.stream, nch, lv, level, part);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlocks), 0, c.stream, part, lv, level, (uint32_t)t.cap,
+                           info);
+        hipLaunchKernelGGL(k_scan_finish, dim3(kScanBlocks), dim3(256), 0, c.stream, nch, lv, level, part, off);
+        hipLaunchKernelGGL(k_make_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level, off,
+                           (uint32_t)t.cap);
         TPM_HIP(hipGetLastError());
-        size_t bytes = c.scan_tmp_bytes;
-        TPM_HIP(rocprim::exclusive_scan(c.scan_tmp_storage, bytes, nch, off, 0u, cnt, rocprim::plus<uint32_t>(),
-                                        c.stream));
-        uint32_t tail[2];
-        TPM_HIP(hipMemcpyAsync(&tail[0], off + cnt - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
-        TPM_HIP(hipMemcpyAsync(&tail[1], nch + cnt - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
-        TPM_HIP(hipStreamSynchronize(c.stream));
-        const size_t added = (size_t)tail[0] + tail[1];
-        grow_tree(c, le + added);
-        hipLaunchKernelGGL(k_make_children, dim3(blocks(cnt, 256)), dim3(256), 0, c.stream, c.keys, t.view(), lb, le, level,
-                           off);
-        TPM_HIP(hipGetLastError());
-        t.count = le + added;
-        lb = le;
-        le = le + added;
-        t.level_begin.push_back(lb);
     }
-    if (t.level_begin.back() != t.count) t.level_begin.push_back(t.count);
-    // Centres of mass, deepest level first.
-    for (int lv = (int)t.level_begin.size() - 2; lv >= 0; --lv) {
-        const size_t b = t.level_begin[lv], e = t.level_begin[lv + 1];
-        if (e <= b) continue;
-        hipLaunchKernelGGL(k_com_level, dim3(blocks(e - b, 256)), dim3(256), 0, c.stream, c.spos, t.view(), b, e);
+    // Centres of mass, deepest level first (levels past the deepest are empty ranges).
+    for (int level = kMaxDepth + 1; level >= 0; --level) {
+        hipLaunchKernelGGL(k_com_level, dim3(grid), dim3(256), 0, c.stream, c.spos, t.view(), lv, level);
         TPM_HIP(hipGetLastError());
     }
 }
@@ -295,11 +359,14 @@ __global__ void k_group_chunk(const uint32_t *__restrict__ segstart, size_t n, u
 }
 
 __global__ void k_group_starts(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ gid, size_t n,
-                               uint32_t *__restrict__ gstart) {
+                               uint32_t *__restrict__ gstart, uint32_t *__restrict__ ngroups) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (flag[i]) gstart[gid[i]] = (uint32_t)i;
-    if (i == n - 1) gstart[gid[i] + flag[i]] = (uint32_t)n;
+    if (i == n - 1) {
+        gstart[gid[i] + flag[i]] = (uint32_t)n;
+        *ngroups = gid[i] + flag[i];
+    }
 }
 
 size_t group_scan_bytes(size_t max_n) {
@@ -314,13 +381,14 @@ size_t group_scan_bytes(size_t max_n) {
 void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
     if (!keys) keys = c.keys;
     if (n == 0) {
-        c.ngroups = 0;
+        zero_fill(c, c.ngroups_dev, sizeof(uint32_t));
         return;
     }
     // Group cubes hold about 512 particles at the mean density (8 particle spacings on a side
     // when mesh = particle grid), and are never larger than L/4.
     int level = 2;
     while (level < kMaxDepth && ((double)n / std::pow(8.0, level + 1)) >= 512.0) ++level;
+    level = std::max(2, std::min(kMaxDepth, level + c.group_level_shift));
     c.group_level = level;
     const int shift = 3 * (kKeyBits - level);
     uint32_t *segval = c.gscratch;
@@ -333,13 +401,9 @@ void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
     TPM_HIP(hipGetLastError());
     bytes = c.gscan_bytes;
     TPM_HIP(rocprim::exclusive_scan(c.gscan_tmp, bytes, segval, tmp, 0u, n, rocprim::plus<uint32_t>(), c.stream));
-    hipLaunchKernelGGL(k_group_starts, dim3(blocks(n, 256)), dim3(256), 0, c.stream, segval, tmp, n, c.gstart);
+    hipLaunchKernelGGL(k_group_starts, dim3(blocks(n, 256)), dim3(256), 0, c.stream, segval, tmp, n, c.gstart,
+                       c.ngroups_dev);
     TPM_HIP(hipGetLastError());
-    uint32_t last[2];
-    TPM_HIP(hipMemcpyAsync(&last[0], tmp + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
-    TPM_HIP(hipMemcpyAsync(&last[1], segval + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
-    TPM_HIP(hipStreamSynchronize(c.stream));
-    c.ngroups = (size_t)last[0] + last[1];
 }
 
 hipError_t rocprim_exclusive_scan_u32(void *tmp, size_t &bytes, const uint32_t *in, uint32_t *out, size_t n,

@@ -53,7 +53,7 @@ struct WalkArgs {
     float tab_scale;  // (1 / (2 r_s)) / (u_cut / kSTab): sqrt(r^2) * tab_scale indexes the S table
     const float2 *stab;
     uint32_t *next_group;
-    uint32_t ngroups;
+    const uint32_t *ngroups;  // device word written by the group builder
     float *acc;
     unsigned long long *counter;
 };
@@ -518,7 +518,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_group_walk(WalkArgs w) {
         uint32_t g = 0;
         if (lane == 0) g = atomicAdd(w.next_group, 1u);
         g = __shfl(g, 0, 64);
-        if (g >= w.ngroups) break;
+        if (g >= *w.ngroups) break;
         walk_group(sh, bs.stab, w, g);
     }
 }
@@ -684,16 +684,18 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     }
     w.gstart = c.gstart;
     w.members = members;
-    w.ngroups = (uint32_t)c.ngroups;
+    w.ngroups = c.ngroups_dev;
     w.next_group = reinterpret_cast<uint32_t *>(c.flag) + 2;
-    TPM_HIP(hipMemsetAsync(w.next_group, 0, sizeof(uint32_t), c.stream));
+    zero_fill(c, w.next_group, sizeof(uint32_t));
     if (c.walk_blocks == 0) {
         int per_cu = 0;
         TPM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_walk, 64 * kWaves, 0));
         c.walk_blocks = (unsigned)std::max(1, per_cu) * (unsigned)c.num_cu;
     }
-    const unsigned nblk = (unsigned)std::min<size_t>(c.walk_blocks, (c.ngroups + kWaves - 1) / kWaves);
-    if (nblk == 0) return;
+    // Persistent grid; workgroups without a group left exit at once. Never more workgroups than
+    // a bound on the group count needs.
+    const size_t bound = ((members ? count : n) + kWaves - 1) / kWaves;
+    const unsigned nblk = (unsigned)std::max<size_t>(1, std::min<size_t>(c.walk_blocks, bound));
     hipLaunchKernelGGL(k_group_walk, dim3(nblk), dim3(64 * kWaves), 0, c.stream, w);
     TPM_HIP(hipGetLastError());
 }
