@@ -118,6 +118,50 @@ def learned_vs_plain(args):
             "error_learned": err_learned}
 
 
+def ensemble_bench(args):
+    """Batched ensemble: independent periodic boxes with sides drawn from {32,48,64,96,128}
+    (mesh = particle grid), per-box seeded Zel'dovich ICs at z = 49 (momenta a^2 f E Psi), the
+    initial force plus KDK steps to z = 9, autotuned, run by one HIP graph launch."""
+    import treepm
+
+    rng = np.random.default_rng(4)
+    sides = [int(v) for v in rng.choice([32, 48, 64, 96, 128], size=args.ensemble_boxes)]
+    total = sum(s ** 3 for s in sides)
+    a0, a1 = 1.0 / (1.0 + Z_INIT), 0.1
+    steps = args.ensemble_steps
+    log(f"[bench] ensemble: {len(sides)} boxes, {total} particles, {steps} KDK steps")
+    t0 = time.perf_counter()
+    e = treepm.Ensemble(sides, box_size=BOX, omega_m=OMEGA_M, softening_scale=1 / 30, theta=args.ensemble_theta)
+    e.ics([1000 + b for b in range(len(sides))], k_c=K_C, disp_cells=0.3,
+          mom_factor=treepm.zeldovich_momentum_factor(a0))
+    setup_s = time.perf_counter() - t0
+    ms_eager_default = e.run(a0, a1, steps, graph=False)
+    log(f"[bench] ensemble: eager run (default knobs) {ms_eager_default:.1f} ms")
+    t0 = time.perf_counter()
+    tuning = e.autotune(a0)
+    tune_s = time.perf_counter() - t0
+    ms_eager = e.run(a0, a1, steps, graph=False)
+    log(f"[bench] ensemble: eager run (tuned) {ms_eager:.1f} ms; tuning {tune_s:.1f} s")
+    t0 = time.perf_counter()
+    e.run(a0, a1, steps, graph=True)
+    capture_s = time.perf_counter() - t0
+    ms_graph = min(e.run(a0, a1, steps, graph=True) for _ in range(2))
+    log(f"[bench] ensemble: graph run {ms_graph:.1f} ms (first launch incl. capture {capture_s:.1f} s)")
+    choices = {}
+    for side, t in zip(sides, tuning):
+        choices[str(side)] = {"fft_cap": t["fft_cap"], "group_level_shift": t["group_level_shift"],
+                              "ms_default": round(t["ms_default"], 3), "ms_tuned": round(t["ms_tuned"], 3)}
+    e.close()
+    evals = total * (steps + 1)
+    return {"workload": f"ensemble_{len(sides)}boxes_zeldovich_z49_to_z9_{steps}kdk",
+            "boxes": len(sides), "sides": {str(v): sides.count(v) for v in sorted(set(sides))},
+            "particles": total, "kdk_steps": steps, "theta": args.ensemble_theta,
+            "force_evals": evals, "ms_graph": ms_graph, "ms_eager_tuned": ms_eager,
+            "ms_eager_default": ms_eager_default, "evals_per_s_graph": evals / (ms_graph / 1000.0),
+            "graph_vs_eager_default": ms_eager_default / ms_graph, "setup_s": setup_s, "tune_s": tune_s,
+            "first_graph_launch_s": capture_s, "tuning": choices}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +180,10 @@ def main():
     ap.add_argument("--learned-samples", type=int, default=20_000_000)
     ap.add_argument("--learned-epochs", type=int, default=3)
     ap.add_argument("--learned-tol", type=float, default=0.005)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-boxes", type=int, default=96)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--ensemble-theta", type=float, default=0.5)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -240,6 +288,12 @@ def main():
                 f"error {lv['error_learned']['rms']:.3e} vs plain {lv['error_plain']['rms']:.3e}")
         except Exception as e:  # reported, not hidden: the line still carries the plain result
             result["learned_vs_plain"] = {"error": repr(e)}
+
+    if not args.no_ensemble:
+        try:
+            result["ensemble"] = ensemble_bench(args)
+        except Exception as e:  # reported, not hidden: the line still carries the plain result
+            result["ensemble"] = {"error": repr(e)}
 
     if not args.no_cpu:
         log("[bench] CPU baseline ...")

@@ -1,0 +1,372 @@
+// Ensembles of independent periodic boxes and the autotuner.
+//
+// No reference counterpart (the reference runs one box): an ensemble holds one context per box
+// (its own mesh, tree scratch and stream) plus device positions, momenta and accelerations.
+// A run (initial force and `steps` KDK steps for every box) is captured once into a HIP graph:
+// the boxes' work is forked onto their own streams from one capture stream and joined back, so
+// one graph launch executes the whole ensemble with the boxes overlapping on the GPU. Every
+// operation of a force evaluation is a kernel launch (no host synchronisation, no memset), which
+// is what makes it capturable.
+//
+// The autotuner times the candidates of two knobs on a box's own input and keeps the fastest:
+// the cap on FFT lines per workgroup (2, 4, 8, 16) and the octree level of the walk groups'
+// cubes (default, one level coarser, one finer). The default is always a candidate, so a tuned
+// box is never slower than the default on the timed input.
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "tpm_internal.h"
+
+namespace tpm {
+
+struct Box {
+    tpm_ctx *ctx = nullptr;
+    int side = 0;
+    size_t n = 0;
+    double *pos = nullptr, *mom = nullptr, *pos0 = nullptr, *mom0 = nullptr;
+    float *acc = nullptr;
+    tpm_tuning tuning{};
+};
+
+}  // namespace tpm
+
+struct tpm_ensemble {
+    tpm_ensemble_params p{};
+    std::vector<tpm::Box> boxes;
+    hipStream_t master = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double g_a0 = 0, g_a1 = 0;
+    int g_steps = -1;
+    std::string err;
+};
+
+// Access to a context's internals (defined in tpm_api.hip).
+tpm::Ctx &tpm_ctx_internal(tpm_ctx *ctx);
+
+namespace tpm {
+
+static double tune_eval_ms(Ctx &c, const double *d_pos, size_t n, double a, float *d_acc, int reps, double *fft_ms,
+                           double *walk_ms) {
+    const bool saved = c.timing;
+    c.timing = true;
+    double best = 1e30, bf = 1e30, bw = 1e30;
+    for (int r = 0; r < reps; ++r) {
+        evaluate(c, d_pos, n, a, false, d_acc, nullptr);
+        best = std::min(best, c.stats.ms_total);
+        bf = std::min(bf, c.stats.ms_fft);
+        bw = std::min(bw, c.stats.ms_walk);
+    }
+    c.timing = saved;
+    if (fft_ms) *fft_ms = bf;
+    if (walk_ms) *walk_ms = bw;
+    return best;
+}
+
+tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_acc) {
+    tpm_tuning t{};
+    const int cap0 = c.fft_cap, shift0 = c.group_level_shift;
+    double f0, w0;
+    t.ms_default = tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f0, &w0);
+    int best_cap = cap0;
+    double best_f = f0;
+    for (int cap : {2, 4, 8, 16}) {
+        if (cap == cap0) continue;
+        c.fft_cap = cap;
+        double f, w;
+        tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f, &w);
+        if (f < best_f) {
+            best_f = f;
+            best_cap = cap;
+        }
+    }
+    c.fft_cap = best_cap;
+    int best_shift = shift0;
+    double best_w = w0;
+    for (int sh : {-1, 1}) {
+        c.group_level_shift = shift0 + sh;
+        double f, w;
+        tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f, &w);
+        if (w < best_w) {
+            best_w = w;
+            best_shift = shift0 + sh;
+        }
+    }
+    c.group_level_shift = best_shift;
+    t.fft_cap = best_cap;
+    t.group_level_shift = best_shift;
+    t.ms_tuned = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr);
+    if (t.ms_tuned > t.ms_default) {
+        // Timing noise can make the tuned choice lose on the final check: keep the default then.
+        c.fft_cap = cap0;
+        c.group_level_shift = shift0;
+        t.fft_cap = cap0;
+        t.group_level_shift = shift0;
+        t.ms_tuned = t.ms_default;
+    }
+    return t;
+}
+
+}  // namespace tpm
+
+using tpm::Ctx;
+
+template <typename F>
+static int eguard(tpm_ensemble *e, F &&f) {
+    if (!e) return TPM_E_INVALID;
+    try {
+        return f(*e);
+    } catch (const tpm::Error &x) {
+        e->err = x.msg;
+        return x.code;
+    }
+}
+
+static void ensemble_free(tpm_ensemble &e) {
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    if (e.graph) (void)hipGraphDestroy(e.graph);
+    for (auto &b : e.boxes) {
+        for (void *p : {(void *)b.pos, (void *)b.mom, (void *)b.pos0, (void *)b.mom0, (void *)b.acc})
+            if (p) (void)hipFree(p);
+        if (b.ctx) tpm_destroy(b.ctx);
+    }
+    if (e.master) (void)hipStreamDestroy(e.master);
+}
+
+extern "C" {
+
+int tpm_ensemble_create(const tpm_ensemble_params *params, int nboxes, const int32_t *sides, tpm_ensemble **out) {
+    if (!params || !sides || !out || nboxes <= 0) return TPM_E_INVALID;
+    *out = nullptr;
+    auto *e = new tpm_ensemble();
+    e->p = *params;
+    try {
+        TPM_HIP(hipStreamCreate(&e->master));
+        e->boxes.resize(nboxes);
+        for (int b = 0; b < nboxes; ++b) {
+            tpm::Box &bx = e->boxes[b];
+            const int s = sides[b];
+            TPM_REQUIRE(s >= 4 && s <= 1024, "box side must be in [4, 1024]");
+            bx.side = s;
+            bx.n = (size_t)s * s * s;
+            // ForceSplit::with(L, s, s, softening_scale, theta) (src/tree.rs:40-50), mesh = particle grid.
+            tpm_params p{};
+            p.mesh_n = s;
+            p.flags = TPM_FLAG_TREEPM;
+            p.box_size = params->box_size;
+            p.omega_m = params->omega_m;
+            // FP32 arithmetic like the reference's ForceSplit.
+            const float cell = (float)params->box_size / (float)s;
+            const float r_split = 1.25f * cell;
+            p.r_split = r_split;
+            p.r_cut = 4.5f * r_split;
+            p.softening = (float)params->softening_scale * cell;
+            p.theta = params->theta;
+            const int rc = tpm_create(&p, bx.n, &bx.ctx);
+            if (rc != TPM_OK) throw tpm::Error{rc, std::string("box context: ") + tpm_last_error(nullptr)};
+            TPM_HIP(hipMalloc(&bx.pos, 3 * bx.n * sizeof(double)));
+            TPM_HIP(hipMalloc(&bx.mom, 3 * bx.n * sizeof(double)));
+            TPM_HIP(hipMalloc(&bx.pos0, 3 * bx.n * sizeof(double)));
+            TPM_HIP(hipMalloc(&bx.mom0, 3 * bx.n * sizeof(double)));
+            TPM_HIP(hipMalloc(&bx.acc, 3 * bx.n * sizeof(float)));
+        }
+    } catch (const tpm::Error &x) {
+        ensemble_free(*e);
+        delete e;
+        return x.code;
+    }
+    *out = e;
+    return TPM_OK;
+}
+
+void tpm_ensemble_destroy(tpm_ensemble *e) {
+    if (!e) return;
+    (void)hipDeviceSynchronize();
+    ensemble_free(*e);
+    delete e;
+}
+
+const char *tpm_ensemble_last_error(const tpm_ensemble *e) { return e ? e->err.c_str() : ""; }
+
+int tpm_ensemble_ics(tpm_ensemble *e, const uint64_t *seeds, double k_c, double disp_cells, double mom_factor) {
+    return eguard(e, [&](tpm_ensemble &E) {
+        TPM_REQUIRE(seeds != nullptr, "null seeds");
+        for (size_t b = 0; b < E.boxes.size(); ++b) {
+            tpm::Box &bx = E.boxes[b];
+            const int rc = tpm_zeldovich_dev(bx.ctx, seeds[b], k_c, disp_cells * E.p.box_size / bx.side, mom_factor,
+                                             bx.pos0, bx.mom0, nullptr);
+            if (rc != TPM_OK) throw tpm::Error{rc, tpm_last_error(bx.ctx)};
+            TPM_HIP(hipStreamSynchronize(tpm_ctx_internal(bx.ctx).stream));
+        }
+        return TPM_OK;
+    });
+}
+
+int tpm_ensemble_autotune(tpm_ensemble *e, double a, tpm_tuning *out) {
+    return eguard(e, [&](tpm_ensemble &E) {
+        // One tuning per distinct box side, on that side's first box; applied to all its boxes.
+        std::vector<int> done;
+        for (size_t b = 0; b < E.boxes.size(); ++b) {
+            tpm::Box &bx = E.boxes[b];
+            Ctx &c = tpm_ctx_internal(bx.ctx);
+            size_t first = b;
+            for (size_t q = 0; q < b; ++q)
+                if (E.boxes[q].side == bx.side) {
+                    first = q;
+                    break;
+                }
+            if (first == b) {
+                TPM_HIP(hipMemcpyAsync(bx.pos, bx.pos0, 3 * bx.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+                bx.tuning = tpm::autotune(c, bx.pos, bx.n, a, bx.acc);
+            } else {
+                bx.tuning = E.boxes[first].tuning;
+                c.fft_cap = bx.tuning.fft_cap;
+                c.group_level_shift = bx.tuning.group_level_shift;
+            }
+            if (out) out[b] = bx.tuning;
+        }
+        // Tuned knobs change the recorded launches: the graph is rebuilt on the next run.
+        if (E.exec) (void)hipGraphExecDestroy(E.exec);
+        if (E.graph) (void)hipGraphDestroy(E.graph);
+        E.exec = nullptr;
+        E.graph = nullptr;
+        return TPM_OK;
+    });
+}
+
+// Enqueues initial force + `steps` KDK steps of every box: eagerly (use_graph = 0, each box on its
+// own stream) or by one launch of the captured graph.
+int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start, double a_end, int steps,
+                     int use_graph, double *ms) {
+    return eguard(e, [&](tpm_ensemble &E) {
+        TPM_REQUIRE(cosmo != nullptr && steps >= 0 && a_start > 0 && a_end > 0, "bad run arguments");
+        // Restore the initial state (outside the graph).
+        for (auto &bx : E.boxes) {
+            Ctx &c = tpm_ctx_internal(bx.ctx);
+            TPM_HIP(hipMemcpyAsync(bx.pos, bx.pos0, 3 * bx.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+            TPM_HIP(hipMemcpyAsync(bx.mom, bx.mom0, 3 * bx.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        }
+        for (auto &bx : E.boxes) TPM_HIP(hipStreamSynchronize(tpm_ctx_internal(bx.ctx).stream));
+        std::vector<double> a(steps + 1);
+        a[0] = a_start;
+        for (int i = 0; i < steps; ++i) a[i + 1] = a_start * std::pow(a_end / a_start, (double)(i + 1) / steps);
+        auto enqueue = [&](tpm::Box &bx) {
+            Ctx &c = tpm_ctx_internal(bx.ctx);
+            tpm::evaluate(c, bx.pos, bx.n, a[0], false, bx.acc, nullptr);
+            for (int i = 0; i < steps; ++i) {
+                const tpm::KdkFactors f = tpm::kdk_factors(*cosmo, a[i], a[i + 1]);
+                tpm::kick(c, bx.mom, bx.acc, bx.n, f.k1);
+                tpm::drift(c, bx.pos, bx.mom, bx.n, f.d);
+                tpm::evaluate(c, bx.pos, bx.n, a[i + 1], false, bx.acc, nullptr);
+                tpm::kick(c, bx.mom, bx.acc, bx.n, f.k2);
+            }
+        };
+        hipEvent_t t0, t1;
+        TPM_HIP(hipEventCreate(&t0));
+        TPM_HIP(hipEventCreate(&t1));
+        if (!use_graph) {
+            TPM_HIP(hipEventRecord(t0, E.master));
+            for (auto &bx : E.boxes) {
+                Ctx &c = tpm_ctx_internal(bx.ctx);
+                TPM_HIP(hipStreamWaitEvent(c.stream, t0, 0));
+                enqueue(bx);
+                hipEvent_t done;
+                TPM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+                TPM_HIP(hipEventRecord(done, c.stream));
+                TPM_HIP(hipStreamWaitEvent(E.master, done, 0));
+                (void)hipEventDestroy(done);
+            }
+            TPM_HIP(hipEventRecord(t1, E.master));
+        } else {
+            if (!E.exec || E.g_a0 != a_start || E.g_a1 != a_end || E.g_steps != steps) {
+                if (E.exec) (void)hipGraphExecDestroy(E.exec);
+                if (E.graph) (void)hipGraphDestroy(E.graph);
+                E.exec = nullptr;
+                E.graph = nullptr;
+                // Warm every box once eagerly so that nothing is allocated during the capture.
+                for (auto &bx : E.boxes) {
+                    Ctx &c = tpm_ctx_internal(bx.ctx);
+                    tpm::evaluate(c, bx.pos, bx.n, a[0], false, bx.acc, nullptr);
+                    TPM_HIP(hipStreamSynchronize(c.stream));
+                }
+                for (auto &bx : E.boxes) {
+                    Ctx &c = tpm_ctx_internal(bx.ctx);
+                    TPM_HIP(hipMemcpyAsync(bx.pos, bx.pos0, 3 * bx.n * sizeof(double), hipMemcpyDeviceToDevice,
+                                           c.stream));
+                    TPM_HIP(hipStreamSynchronize(c.stream));
+                }
+                hipEvent_t fork;
+                TPM_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+                TPM_HIP(hipStreamBeginCapture(E.master, hipStreamCaptureModeRelaxed));
+                TPM_HIP(hipEventRecord(fork, E.master));
+                std::vector<hipEvent_t> joins;
+                for (auto &bx : E.boxes) {
+                    Ctx &c = tpm_ctx_internal(bx.ctx);
+                    TPM_HIP(hipStreamWaitEvent(c.stream, fork, 0));
+                    enqueue(bx);
+                    hipEvent_t done;
+                    TPM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+                    TPM_HIP(hipEventRecord(done, c.stream));
+                    TPM_HIP(hipStreamWaitEvent(E.master, done, 0));
+                    joins.push_back(done);
+                }
+                TPM_HIP(hipStreamEndCapture(E.master, &E.graph));
+                for (auto ev : joins) (void)hipEventDestroy(ev);
+                (void)hipEventDestroy(fork);
+                TPM_HIP(hipGraphInstantiate(&E.exec, E.graph, nullptr, nullptr, 0));
+                E.g_a0 = a_start;
+                E.g_a1 = a_end;
+                E.g_steps = steps;
+            }
+            TPM_HIP(hipEventRecord(t0, E.master));
+            TPM_HIP(hipGraphLaunch(E.exec, E.master));
+            TPM_HIP(hipEventRecord(t1, E.master));
+        }
+        TPM_HIP(hipEventSynchronize(t1));
+        for (auto &bx : E.boxes) tpm::finish_checks(tpm_ctx_internal(bx.ctx));
+        float t = 0.f;
+        TPM_HIP(hipEventElapsedTime(&t, t0, t1));
+        if (ms) *ms = t;
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        return TPM_OK;
+    });
+}
+
+int tpm_ensemble_get(tpm_ensemble *e, int box, double *pos, double *mom, float *acc) {
+    return eguard(e, [&](tpm_ensemble &E) {
+        TPM_REQUIRE(box >= 0 && box < (int)E.boxes.size(), "box index out of range");
+        tpm::Box &bx = E.boxes[box];
+        TPM_HIP(hipDeviceSynchronize());
+        if (pos) TPM_HIP(hipMemcpy(pos, bx.pos, 3 * bx.n * sizeof(double), hipMemcpyDeviceToHost));
+        if (mom) TPM_HIP(hipMemcpy(mom, bx.mom, 3 * bx.n * sizeof(double), hipMemcpyDeviceToHost));
+        if (acc) TPM_HIP(hipMemcpy(acc, bx.acc, 3 * bx.n * sizeof(float), hipMemcpyDeviceToHost));
+        return TPM_OK;
+    });
+}
+
+int tpm_autotune(tpm_ctx *ctx, const double *d_pos, size_t n, double a, float *d_acc, tpm_tuning *out) {
+    if (!ctx || !d_pos || !d_acc || n == 0) return TPM_E_INVALID;
+    Ctx &c = tpm_ctx_internal(ctx);
+    try {
+        const tpm_tuning t = tpm::autotune(c, d_pos, n, a, d_acc);
+        if (out) *out = t;
+    } catch (const tpm::Error &x) {
+        c.err = x.msg;
+        return x.code;
+    }
+    return TPM_OK;
+}
+
+int tpm_set_tuning(tpm_ctx *ctx, const tpm_tuning *t) {
+    if (!ctx || !t) return TPM_E_INVALID;
+    Ctx &c = tpm_ctx_internal(ctx);
+    if (t->fft_cap < 1 || t->fft_cap > 64) return TPM_E_INVALID;
+    c.fft_cap = t->fft_cap;
+    c.group_level_shift = t->group_level_shift;
+    return TPM_OK;
+}
+
+}  // extern "C"

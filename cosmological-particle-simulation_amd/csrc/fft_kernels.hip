@@ -486,7 +486,7 @@ static void pm_pipeline(Ctx &c, const ZSrc<R> &src, typename Cx<R>::T *SA, typen
 
     // Z forward
     {
-        const int nb = lines_per_block(n, es, 1, 2, 16);
+        const int nb = lines_per_block(n, es, 1, 2, c.fft_cap);
         const size_t lines = (size_t)n * n;
         dim3 grid((unsigned)((lines + nb - 1) / nb));
         const size_t sh = 2 * (size_t)nb * ld * es;
@@ -495,7 +495,7 @@ static void pm_pipeline(Ctx &c, const ZSrc<R> &src, typename Cx<R>::T *SA, typen
     }
     // Y forward
     {
-        const int nb = lines_per_block(n, es, 1, 2, 16);
+        const int nb = lines_per_block(n, es, 1, 2, c.fft_cap);
         dim3 grid((unsigned)((hp + nb - 1) / nb), (unsigned)n);
         const size_t sh = 2 * (size_t)nb * ld * es;
         hipLaunchKernelGGL(k_fft_y<R>, grid, dim3(256), sh, c.stream, SA, (C *)nullptr, pl, tw,
@@ -504,7 +504,7 @@ static void pm_pipeline(Ctx &c, const ZSrc<R> &src, typename Cx<R>::T *SA, typen
     }
     // X forward + Green + 2 inverse
     {
-        const int nb = lines_per_block(n, es, 2, 2, 16);
+        const int nb = lines_per_block(n, es, 2, 2, c.fft_cap);
         dim3 grid((unsigned)((hp + nb - 1) / nb), (unsigned)n);
         const size_t sh = 4 * (size_t)nb * ld * es;
         TPM_REQUIRE(sh <= 65536, "mesh too large for the fused x pass");
@@ -514,13 +514,13 @@ static void pm_pipeline(Ctx &c, const ZSrc<R> &src, typename Cx<R>::T *SA, typen
     }
     // Y inverse: A in place; B -> B, C
     {
-        const int nb = lines_per_block(n, es, 1, 2, 16);
+        const int nb = lines_per_block(n, es, 1, 2, c.fft_cap);
         dim3 grid((unsigned)((hp + nb - 1) / nb), (unsigned)n);
         const size_t sh = 2 * (size_t)nb * ld * es;
         hipLaunchKernelGGL(k_fft_y<R>, grid, dim3(256), sh, c.stream, SA, (C *)nullptr, pl, tw,
                            (const double *)c.kvec, nb, hp, 1);
         TPM_HIP(hipGetLastError());
-        const int nb2 = lines_per_block(n, es, 2, 2, 16);
+        const int nb2 = lines_per_block(n, es, 2, 2, c.fft_cap);
         dim3 grid2((unsigned)((hp + nb2 - 1) / nb2), (unsigned)n);
         const size_t sh2 = 4 * (size_t)nb2 * ld * es;
         TPM_REQUIRE(sh2 <= 65536, "mesh too large for the split y pass");
@@ -530,7 +530,7 @@ static void pm_pipeline(Ctx &c, const ZSrc<R> &src, typename Cx<R>::T *SA, typen
     }
     // Z inverse (c2r) of the three components
     {
-        const int nb = lines_per_block(n, es, 3, 2, 16);
+        const int nb = lines_per_block(n, es, 3, 2, c.fft_cap);
         const size_t sh = 6 * (size_t)nb * ld * es;
         TPM_REQUIRE(sh <= 65536, "mesh too large for the c2r pass");
         const size_t lines = (size_t)n * n;

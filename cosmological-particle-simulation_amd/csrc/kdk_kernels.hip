@@ -57,6 +57,56 @@ KdkFactors kdk_factors(const tpm_cosmology &c, double a0, double a1) {
     return k;
 }
 
+// Linear growth state (delta, d delta / d ln a) at a_end: the growth ODE in ln a,
+//   delta'' + (2 + dlnE/dlna) delta' = 1.5 Om delta / (a^3 E^2),
+// RK4 with 4000 steps from a = 1e-3, started on the Meszaros growing mode 1 + 1.5 y (y = a Om/Or)
+// with radiation, or delta = a without (Cosmology::growth_state, src/cosmology.rs:41-79), in FP64.
+static void growth_state(const tpm_cosmology &c, double a_end, double *d_out, double *dl_out) {
+    const double a_start = 1e-3;
+    double d, dl;
+    if (c.omega_r > 0.0) {
+        const double y = a_start * c.omega_m / c.omega_r;
+        d = 1.0 + 1.5 * y;
+        dl = 1.5 * y;
+    } else {
+        d = a_start;
+        dl = a_start;
+    }
+    auto rhs = [&](double la, double x, double xl, double *fx, double *fxl) {
+        const double a = std::exp(la);
+        const double a3 = a * a * a, a4 = a3 * a;
+        const double e2 = c.omega_r / a4 + c.omega_m / a3 + c.omega_l;
+        const double dln_e = 0.5 * (-4.0 * c.omega_r / a4 - 3.0 * c.omega_m / a3) / e2;
+        *fx = xl;
+        *fxl = -(2.0 + dln_e) * xl + 1.5 * c.omega_m / (a3 * e2) * x;
+    };
+    const int steps = 4000;
+    const double la0 = std::log(a_start);
+    const double h = (std::log(a_end) - la0) / steps;
+    for (int s = 0; s < steps; ++s) {
+        const double la = la0 + s * h;
+        double k1, l1, k2, l2, k3, l3, k4, l4;
+        rhs(la, d, dl, &k1, &l1);
+        rhs(la + 0.5 * h, d + 0.5 * h * k1, dl + 0.5 * h * l1, &k2, &l2);
+        rhs(la + 0.5 * h, d + 0.5 * h * k2, dl + 0.5 * h * l2, &k3, &l3);
+        rhs(la + h, d + h * k3, dl + h * l3, &k4, &l4);
+        d += h / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
+        dl += h / 6.0 * (l1 + 2.0 * l2 + 2.0 * l3 + l4);
+    }
+    *d_out = d;
+    *dl_out = dl;
+}
+
+void growth(const tpm_cosmology &c, double a, double *d, double *f) {
+    double d1, dl1, da, dla;
+    growth_state(c, 1.0, &d1, &dl1);
+    growth_state(c, a, &da, &dla);
+    if (d) *d = da / d1;
+    if (f) *f = dla / da;
+}
+
+double hubble_e(const tpm_cosmology &c, double a) { return cosmo_e(c, a); }
+
 static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 void kick(Ctx &c, double *d_mom, const float *d_acc, size_t n, double f) {

@@ -163,7 +163,7 @@ static void read_tree_stats(Ctx &c, bool built) {
 }
 
 // The force evaluation on device-resident positions (caller order, FP64).
-static void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float *d_acc, float *d_delta) {
+void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float *d_acc, float *d_delta) {
     TPM_REQUIRE(n <= c.max_n, "more particles than the context capacity");
     TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
     const bool treepm = (c.p.flags & TPM_FLAG_TREEPM) != 0;
@@ -211,7 +211,7 @@ static void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w,
     }
 }
 
-static void finish_checks(Ctx &c) {
+void finish_checks(Ctx &c) {
     unsigned long long ovf = 0;
     uint32_t info[2] = {0, 0};
     TPM_HIP(hipMemcpyAsync(&ovf, c.counter + 1, sizeof(ovf), hipMemcpyDeviceToHost, c.stream));
@@ -259,6 +259,8 @@ using tpm::Ctx;
 struct tpm_ctx {
     Ctx c;
 };
+
+tpm::Ctx &tpm_ctx_internal(tpm_ctx *ctx) { return ctx->c; }
 
 template <typename F>
 static int guarded(tpm_ctx *ctx, F &&f) {
@@ -769,6 +771,13 @@ int tpm_kdk_factors(const tpm_cosmology *cosmo, double a0, double a1, double *am
     if (k1) *k1 = f.k1;
     if (d) *d = f.d;
     if (k2) *k2 = f.k2;
+    return TPM_OK;
+}
+
+int tpm_growth(const tpm_cosmology *cosmo, double a, double *d, double *f, double *e) {
+    if (!cosmo || !(a > 0) || !std::isfinite(a)) return TPM_E_INVALID;
+    tpm::growth(*cosmo, a, d, f);
+    if (e) *e = tpm::hubble_e(*cosmo, a);
     return TPM_OK;
 }
 

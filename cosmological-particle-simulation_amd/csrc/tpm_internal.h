@@ -144,6 +144,7 @@ struct Ctx {
     size_t gscan_bytes = 0;
     int group_level = 2;
     int group_level_shift = 0;  // autotuned offset of the group cube level
+    int fft_cap = 16;           // autotuned cap on FFT lines per workgroup
     uint32_t *tree_scan = nullptr;  // per-level child counts, offsets, scan partials, level table
     uint32_t *tree_info = nullptr;  // [0] node count, [1] capacity exceeded
     uint32_t *ngroups_dev = nullptr;  // walk group count (device)
@@ -198,6 +199,13 @@ void walk_judge_f64(Ctx &c, size_t n, const uint32_t *d_targets_sorted, size_t m
 // FP64 interpolation of c.dforce at listed sorted indices.
 void interp_judge_f64(Ctx &c, const uint32_t *d_targets_sorted, size_t m, double *d_out);
 
+// Force evaluation on device-resident FP64 positions (tpm_api.hip); no host synchronisation
+// unless timing is on.
+void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float *d_acc, float *d_delta);
+// Synchronises c.stream and throws on a walk stack overflow, octree capacity overflow or (with
+// TPM_FLAG_CHECK_FINITE) a non-finite acceleration of the last evaluation.
+void finish_checks(Ctx &c);
+
 // Misc.
 // Zero `bytes` (a multiple of 4) at ptr with a fill kernel (graph-capture safe, unlike memset).
 void zero_fill(Ctx &c, void *ptr, size_t bytes);
@@ -217,6 +225,10 @@ struct KdkFactors {
     double am, k1, d, k2;
 };
 KdkFactors kdk_factors(const tpm_cosmology &c, double a0, double a1);
+// Cosmology::growth (D, normalised to D(1) = 1) and growth_rate f = dlnD/dlna, FP64.
+void growth(const tpm_cosmology &c, double a, double *d, double *f);
+// E(a) = H(a)/H0.
+double hubble_e(const tpm_cosmology &c, double a);
 void kick(Ctx &c, double *d_mom, const float *d_acc, size_t n, double f);
 void drift(Ctx &c, double *d_pos, const double *d_mom, size_t n, double f);
 

@@ -45,6 +45,8 @@ EXPORTS = [
     "tpm_device_alloc", "tpm_device_free", "tpm_memcpy", "tpm_sync", "tpm_nfw_halos", "tpm_nfw_halos_dev",
     "tpm_mlp_param_count", "tpm_mlp_init", "tpm_mlp_get_params", "tpm_mlp_set_params", "tpm_mlp_gradients",
     "tpm_mlp_predict", "tpm_mlp_train_dev", "tpm_accelerations_learned_dev",
+    "tpm_growth", "tpm_autotune", "tpm_set_tuning", "tpm_ensemble_create", "tpm_ensemble_destroy", "tpm_ensemble_last_error",
+    "tpm_ensemble_ics", "tpm_ensemble_autotune", "tpm_ensemble_run", "tpm_ensemble_get",
 ]
 
 
@@ -116,6 +118,19 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("fft_cap", ctypes.c_int32), ("group_level_shift", ctypes.c_int32),
+                ("ms_default", ctypes.c_double), ("ms_tuned", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class EnsembleParams(ctypes.Structure):
+    _fields_ = [("box_size", ctypes.c_double), ("omega_m", ctypes.c_double),
+                ("softening_scale", ctypes.c_double), ("theta", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -166,6 +181,18 @@ def load():
         "tpm_mlp_predict": (ctypes.c_int, [P, P, S, P]),
         "tpm_mlp_train_dev": (ctypes.c_int, [P, P, S, D, S, ctypes.c_int, S, F, ctypes.c_uint64, P]),
         "tpm_accelerations_learned_dev": (ctypes.c_int, [P, P, S, D, D, P, ctypes.POINTER(ctypes.c_uint64)]),
+        "tpm_growth": (ctypes.c_int, [ctypes.POINTER(Cosmology), D, P, P, P]),
+        "tpm_autotune": (ctypes.c_int, [P, P, S, D, P, ctypes.POINTER(Tuning)]),
+        "tpm_set_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
+        "tpm_ensemble_create": (ctypes.c_int, [ctypes.POINTER(EnsembleParams), ctypes.c_int, P,
+                                               ctypes.POINTER(ctypes.c_void_p)]),
+        "tpm_ensemble_destroy": (None, [P]),
+        "tpm_ensemble_last_error": (ctypes.c_char_p, [P]),
+        "tpm_ensemble_ics": (ctypes.c_int, [P, P, D, D, D]),
+        "tpm_ensemble_autotune": (ctypes.c_int, [P, D, P]),
+        "tpm_ensemble_run": (ctypes.c_int, [P, ctypes.POINTER(Cosmology), D, D, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_double)]),
+        "tpm_ensemble_get": (ctypes.c_int, [P, ctypes.c_int, P, P, P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -405,6 +432,18 @@ class Solver:
     def sync(self):
         self._check(self.lib.tpm_sync(self.h))
 
+    def autotune(self, d_pos, n, a, d_acc):
+        """Times the FFT and walk-group knobs on this input and keeps the fastest (returns the choice)."""
+        t = Tuning()
+        self._check(self.lib.tpm_autotune(self.h, ctypes.c_void_p(d_pos), int(n), float(a), ctypes.c_void_p(d_acc),
+                                          ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, fft_cap=16, group_level_shift=0):
+        t = Tuning()
+        t.fft_cap, t.group_level_shift = int(fft_cap), int(group_level_shift)
+        self._check(self.lib.tpm_set_tuning(self.h, ctypes.byref(t)))
+
     def judge(self, pos, idx, a, theta=0.2):
         """FP64 TreePM force (FP64 PM + FP64 per-particle walk at `theta`) at caller indices idx."""
         pos = _f64(pos, (-1, 3))
@@ -413,6 +452,67 @@ class Solver:
         self._check(self.lib.tpm_judge_f64(self.h, _ptr(pos), len(pos), _ptr(idx), len(idx), float(a),
                                            float(theta), _ptr(out)))
         return out
+
+
+class Ensemble:
+    """Independent periodic boxes (box b: sides[b]^3 particles on a sides[b]^3 mesh), each run
+    from its own Zel'dovich ICs through KDK steps; a run can be one HIP graph launch."""
+
+    def __init__(self, sides, box_size=100.0, omega_m=None, softening_scale=1 / 30, theta=0.5):
+        self.lib = load()
+        self.sides = [int(s) for s in sides]
+        p = EnsembleParams()
+        p.box_size = float(box_size)
+        p.omega_m = float(Cosmology.lcdm_default().omega_m if omega_m is None else omega_m)
+        p.softening_scale = float(softening_scale)
+        p.theta = float(theta)
+        arr = np.ascontiguousarray(self.sides, dtype=np.int32)
+        h = ctypes.c_void_p()
+        rc = self.lib.tpm_ensemble_create(ctypes.byref(p), len(arr), _ptr(arr), ctypes.byref(h))
+        if rc != TPM_OK:
+            raise TpmError(rc, self.lib.tpm_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != TPM_OK:
+            raise TpmError(rc, self.lib.tpm_ensemble_last_error(self.h).decode())
+
+    def ics(self, seeds, k_c=0.25, disp_cells=0.3, mom_factor=0.0):
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        assert len(seeds) == len(self.sides)
+        self._check(self.lib.tpm_ensemble_ics(self.h, _ptr(seeds), float(k_c), float(disp_cells), float(mom_factor)))
+
+    def autotune(self, a):
+        out = (Tuning * len(self.sides))()
+        self._check(self.lib.tpm_ensemble_autotune(self.h, float(a), out))
+        return [t.as_dict() for t in out]
+
+    def run(self, a_start, a_end, steps, graph=True, cosmo=None):
+        """Initial force plus `steps` KDK steps of every box; returns the run's device milliseconds."""
+        cosmo = cosmo or Cosmology.lcdm_default()
+        ms = ctypes.c_double()
+        self._check(self.lib.tpm_ensemble_run(self.h, ctypes.byref(cosmo), float(a_start), float(a_end), int(steps),
+                                              1 if graph else 0, ctypes.byref(ms)))
+        return ms.value
+
+    def get(self, b):
+        n = self.sides[b] ** 3
+        pos = np.empty((n, 3), np.float64)
+        mom = np.empty((n, 3), np.float64)
+        acc = np.empty((n, 3), np.float32)
+        self._check(self.lib.tpm_ensemble_get(self.h, int(b), _ptr(pos), _ptr(mom), _ptr(acc)))
+        return pos, mom, acc
+
+    def close(self):
+        if self.h:
+            self.lib.tpm_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def coupling(omega_m, box_size, a, n):
@@ -430,6 +530,22 @@ def kdk_factors(a0, a1, cosmo=None):
     if rc != TPM_OK:
         raise TpmError(rc, "invalid KDK factor arguments")
     return tuple(o.value for o in out)
+
+
+def growth(a, cosmo=None):
+    """(D(a), f(a), E(a)): Cosmology::growth, growth_rate and e (src/cosmology.rs:31-91), FP64."""
+    cosmo = cosmo or Cosmology.lcdm_default()
+    out = [ctypes.c_double() for _ in range(3)]
+    rc = load().tpm_growth(ctypes.byref(cosmo), float(a), *[ctypes.byref(o) for o in out])
+    if rc != TPM_OK:
+        raise TpmError(rc, "invalid growth arguments")
+    return tuple(o.value for o in out)
+
+
+def zeldovich_momentum_factor(a, cosmo=None):
+    """p = a^2 f(a) E(a) x displacement at a (src/ics.rs:155-160, the displacement already at a)."""
+    _, f, e = growth(a, cosmo)
+    return a * a * f * e
 
 
 class DeviceArray:

@@ -225,11 +225,61 @@ typedef struct tpm_cosmology {
 } tpm_cosmology;
 
 /*
+ * Autotuning (no reference counterpart): times the candidates of two knobs on the given input and
+ * keeps the fastest (the default is a candidate): the cap on FFT lines per workgroup and the
+ * octree level of the walk groups' cubes (as an offset from the default level).
+ */
+typedef struct tpm_tuning {
+    int32_t fft_cap;
+    int32_t group_level_shift;
+    double ms_default; /* force evaluation with the default knobs */
+    double ms_tuned;   /* with the chosen knobs */
+} tpm_tuning;
+
+int tpm_autotune(tpm_ctx *ctx, const double *d_pos, size_t n, double a, float *d_acc, tpm_tuning *out);
+int tpm_set_tuning(tpm_ctx *ctx, const tpm_tuning *tuning);
+
+/*
+ * Ensemble of independent periodic boxes (no reference counterpart): box b has sides[b]^3
+ * particles on a sides[b]^3 mesh, the split of ForceSplit::with (src/tree.rs:40-50) with the given
+ * softening (in cells) and theta. A run is the initial force and `steps` KDK steps of every box;
+ * with use_graph it is captured once into a HIP graph (boxes forked onto their own streams) and
+ * executed by one graph launch. ms receives the run's device time.
+ */
+typedef struct tpm_ensemble_params {
+    double box_size;
+    double omega_m;
+    double softening_scale; /* softening in mesh cells */
+    double theta;
+} tpm_ensemble_params;
+
+typedef struct tpm_ensemble tpm_ensemble;
+
+int tpm_ensemble_create(const tpm_ensemble_params *params, int nboxes, const int32_t *sides, tpm_ensemble **out);
+void tpm_ensemble_destroy(tpm_ensemble *ens);
+const char *tpm_ensemble_last_error(const tpm_ensemble *ens);
+/* Lattice Zel'dovich ICs per box (seeds[b]; displacement RMS in cells; momenta = mom_factor Psi). */
+int tpm_ensemble_ics(tpm_ensemble *ens, const uint64_t *seeds, double k_c, double disp_cells, double mom_factor);
+/* Autotunes each distinct box side on its first box; out (nboxes entries, may be NULL). */
+int tpm_ensemble_autotune(tpm_ensemble *ens, double a, tpm_tuning *out);
+int tpm_ensemble_run(tpm_ensemble *ens, const tpm_cosmology *cosmo, double a_start, double a_end, int steps,
+                     int use_graph, double *ms);
+/* Copies box `box`'s current positions, momenta (3 side^3 doubles each) and accelerations (floats). */
+int tpm_ensemble_get(tpm_ensemble *ens, int box, double *pos, double *mom, float *acc);
+
+/*
  * KDK step factors of Simulation::step_to (src/integrator.rs:174-186, 262-297) in FP64: the time
  * midpoint am, the half-kick integrals k1 = int_{a0}^{am} da/(aE), k2 = int_{am}^{a1} da/(aE) and
  * the drift integral d = int_{a0}^{a1} da/(a^3 E). No device work.
  */
 int tpm_kdk_factors(const tpm_cosmology *cosmo, double a0, double a1, double *am, double *k1, double *d, double *k2);
+
+/*
+ * Linear growth D(a) (D(1) = 1), growth rate f = dlnD/dlna (Cosmology::growth, growth_rate:
+ * src/cosmology.rs:41-91, the RK4 growth ODE in FP64) and E(a) = H/H0 (src/cosmology.rs:31-33).
+ * Zel'dovich momenta are p = a^2 f E x displacement (src/ics.rs:155-160). No device work.
+ */
+int tpm_growth(const tpm_cosmology *cosmo, double a, double *d, double *f, double *e);
 
 /*
  * One KDK step, Simulation::step_to (src/integrator.rs:174-186), on device arrays: p += acc k1;

@@ -1,0 +1,122 @@
+"""Ensembles of independent boxes run by one HIP graph launch, and the autotuner.
+
+A box of the ensemble runs the same kernels as a single context's `tpm_run` (src/integrator.rs:163-316
+driven from lattice Zel'dovich ICs), so with the default knobs its final state must equal that run's
+bit for bit; graph replays must equal the eager run bit for bit, replay after replay, with eager work
+and synchronisations between them."""
+import numpy as np
+import pytest
+
+from helpers import OMEGA_M
+
+pytestmark = pytest.mark.gpu
+
+SIDES = [16, 24, 32, 16, 24]
+SEEDS = [11, 12, 13, 14, 15]
+A0, A1, STEPS = 0.02, 0.05, 4
+MOM = 0.02 * 0.5  # momenta = mom_factor * displacement (any fixed value exercises the kicks)
+
+
+def _ens(treepm):
+    print(f"[gpu] ensemble sides={SIDES}", flush=True)
+    e = treepm.Ensemble(SIDES, box_size=100.0, omega_m=OMEGA_M, softening_scale=1 / 30, theta=0.5)
+    e.ics(SEEDS, k_c=0.25, disp_cells=0.3, mom_factor=MOM)
+    return e
+
+
+def _states(e):
+    return [e.get(b) for b in range(len(SIDES))]
+
+
+def test_graph_replays_equal_eager_run(treepm):
+    e = _ens(treepm)
+    e.run(A0, A1, STEPS, graph=False)
+    eager = _states(e)
+    for b, (p, m, a) in enumerate(eager):
+        assert np.isfinite(p).all() and np.isfinite(m).all() and np.isfinite(a).all(), b
+        assert (p >= 0).all() and (p < 100.0).all(), b
+    for rep in range(3):
+        print(f"[gpu] graph replay {rep}", flush=True)
+        e.run(A0, A1, STEPS, graph=True)
+        got = _states(e)
+        for b in range(len(SIDES)):
+            for x, y in zip(got[b], eager[b]):
+                assert np.array_equal(x, y), (rep, b)
+        # eager work between replays
+        e.run(A0, A1, 1, graph=False)
+    e.close()
+
+
+def test_box_equals_single_context_run(treepm):
+    e = _ens(treepm)
+    e.run(A0, A1, STEPS, graph=True)
+    for b in (0, 2):
+        side = SIDES[b]
+        split = treepm.force_split(100.0, side, side, 1 / 30, 0.5)
+        s = treepm.Solver(side, 100.0, OMEGA_M, split=split, max_particles=side ** 3)
+        pos, mom = s.zeldovich(SEEDS[b], 0.25, 0.3 * 100.0 / side, MOM, want_momenta=True)
+        p1, m1 = s.run(pos, mom, A0, A1, STEPS)
+        pe, me, _ = e.get(b)
+        assert np.array_equal(pe, p1), b
+        assert np.array_equal(me, m1), b
+    e.close()
+
+
+def test_autotune_keeps_results_and_is_not_slower(treepm):
+    e = _ens(treepm)
+    e.run(A0, A1, 0, graph=False)  # the initial force only
+    ref = _states(e)
+    choices = e.autotune(A0)
+    for t in choices:
+        print(f"[gpu] tuning {t}", flush=True)
+        assert t["fft_cap"] in (2, 4, 8, 16)
+        assert t["group_level_shift"] in (-1, 0, 1)
+        assert t["ms_tuned"] <= t["ms_default"]
+    e.run(A0, A1, 0, graph=True)
+    got = _states(e)
+    for b in range(len(SIDES)):
+        # another group level changes which nodes the walk accepts (an error at the theta = 0.5
+        # level, a few 1e-3 of the force RMS) and the summation order, nothing else
+        a0, a1 = ref[b][2].astype(np.float64), got[b][2].astype(np.float64)
+        rel = np.sqrt(np.mean(np.sum((a1 - a0) ** 2, axis=1)) / np.mean(np.sum(a0 ** 2, axis=1)))
+        print(f"[gpu] box {b} tuned-vs-default force rms {rel:.2e}", flush=True)
+        assert rel < 2e-2, b
+    e.run(A0, A1, STEPS, graph=True)
+    for p, m, a in _states(e):
+        assert np.isfinite(p).all() and np.isfinite(a).all()
+    e.close()
+
+
+def test_solver_autotune_and_set_tuning(treepm):
+    side = 32
+    split = treepm.force_split(100.0, side, side, 1 / 30, 0.5)
+    s = treepm.Solver(side, 100.0, OMEGA_M, split=split, max_particles=side ** 3)
+    pos, _ = s.zeldovich(3, 0.25, 0.3 * 100.0 / side)
+    d_pos = treepm.DeviceArray.from_numpy(pos)
+    d_acc = treepm.DeviceArray(3 * side ** 3, np.float32)
+    s.set_tuning(16, 0)
+    s.accelerations_dev(d_pos.ptr, side ** 3, 0.5, d_acc.ptr)
+    s.sync()
+    base = d_acc.numpy()
+    s.set_tuning(4, 0)  # FFT batching only: results unchanged bit for bit
+    s.accelerations_dev(d_pos.ptr, side ** 3, 0.5, d_acc.ptr)
+    s.sync()
+    assert np.array_equal(d_acc.numpy(), base)
+    # every group level the tuner may pick keeps the force error at the default's level (FP64 judge)
+    idx = np.arange(0, side ** 3, 7, dtype=np.uint32)
+    ref = s.judge(pos, idx, 0.5)
+    scale = np.sqrt(np.mean(np.sum(ref ** 2, axis=1)))
+    errs = {}
+    for shift in (-1, 0, 1):
+        s.set_tuning(16, shift)
+        s.accelerations_dev(d_pos.ptr, side ** 3, 0.5, d_acc.ptr)
+        s.sync()
+        got = d_acc.numpy().reshape(-1, 3)[idx].astype(np.float64)
+        errs[shift] = np.sqrt(np.mean(np.sum((got - ref) ** 2, axis=1))) / scale
+    print(f"[gpu] force rms error vs FP64 judge by group level shift: {errs}", flush=True)
+    assert max(errs.values()) < 1.5 * errs[0] + 1e-3
+    s.set_tuning(16, 0)
+    t = s.autotune(d_pos.ptr, side ** 3, 0.5, d_acc.ptr)
+    assert t.ms_tuned <= t.ms_default
+    with pytest.raises(treepm.TpmError):
+        s.set_tuning(0, 0)

@@ -78,3 +78,14 @@ def test_oracle_steps_converge_and_conserve_momentum(oracle):
     p, m = oracle.sim_run(p0, m0, 32, 1.0, 0.02, 0.1, 20)
     scale = np.abs(m[:, 0]).sum()
     assert np.all(np.abs(m.astype(np.float64).sum(0)) < 1e-3 * scale)
+
+
+@pytest.mark.parametrize("a", [0.02, 0.1, 0.5, 1.0])
+def test_library_growth_matches_oracle(oracle, treepm, a):
+    # tpm_growth is the host-side FP64 restatement of Cosmology::growth / growth_rate
+    # (src/cosmology.rs:41-91); the oracle's FP64 twin is pinned by D(0.5) and f = dlnD/dlna above.
+    d, f, e = treepm.growth(a)
+    assert abs(d - oracle.growth(a, dtype=np.float64)) < 1e-12
+    assert abs(f - oracle.growth_rate(a, dtype=np.float64)) < 1e-12
+    assert abs(e - oracle.cosmo_e(a, dtype=np.float64)) < 1e-12 * e
+    assert treepm.zeldovich_momentum_factor(a) == pytest.approx(a * a * f * e, rel=1e-15)
