@@ -11,6 +11,7 @@ Prints one JSON line: particle-force evaluations/s, force error vs an FP64 TreeP
 CPU baseline (the oracle's restatement of the reference algorithm on this host's cores).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -180,11 +181,24 @@ def main():
     ap.add_argument("--learned-samples", type=int, default=20_000_000)
     ap.add_argument("--learned-epochs", type=int, default=3)
     ap.add_argument("--learned-tol", type=float, default=0.005)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default knobs)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-boxes", type=int, default=96)
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--ensemble-theta", type=float, default=0.5)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device three times (calibrates HBM byte counters)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import treepm
+        src = treepm.DeviceArray(args.copy_probe, np.uint8)
+        dst = treepm.DeviceArray(args.copy_probe, np.uint8)
+        for _ in range(3):
+            if treepm.load().tpm_memcpy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), args.copy_probe, 2) != 0:
+                raise RuntimeError("device copy failed")
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "copies": 3}), flush=True)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -212,6 +226,11 @@ def main():
         solver.accelerations_dev(pos.ptr, npart, a, acc.ptr)
         solver.sync()
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    tuning = None
+    if not args.no_tune:
+        tuning = solver.autotune(pos.ptr, npart, a, acc.ptr).as_dict()
+        solver.sync()
+        log(f"[bench] autotune: {tuning}")
 
     solver.set_timing(True)
     stats = []
@@ -252,6 +271,7 @@ def main():
                                 else f"treepm_nfw20000_{n}cubed_mesh{n}_z0"), "particles": npart, "mesh": n,
                    "box": BOX, "theta": args.theta, "r_split_cells": 1.25, "r_cut_over_r_split": 4.5,
                    "softening": split["softening"], "positions": "f64", "pair_and_mesh_arithmetic": "f32"},
+        "tuning": tuning,
         "stages_ms": stages,
         "interactions_per_particle": inter / npart,
         "walk_sources_per_group": {"groups": mean("groups"), "listed": mean("listed") / max(mean("groups"), 1),

@@ -24,18 +24,33 @@ struct Stencil {
     T w[8];
 };
 
+// Periodic wrap of an unwrapped cell index (rem_euclid). For positions in [0, L) the stencil
+// corners lie in [-1, n], so one compare-and-add covers them; anything else takes the modulo.
+__device__ __forceinline__ int64_t wrap_cell(int64_t v, int n) {
+    if (v >= 0 && v < n) return v;
+    if (v == -1) return n - 1;
+    if (v == n) return 0;
+    const int64_t r = v % n;
+    return r < 0 ? r + n : r;
+}
+
+// Stencil base (unwrapped) and fraction on one axis: s = x / h - 0.5, base = floor(s).
 template <typename T>
-__device__ __forceinline__ void cic_stencil(T x0, T x1, T x2, int n, T box, Stencil<T> &st) {
-    const T h = box / (T)n;
-    const T xs[3] = {x0, x1, x2};
-    int64_t base[3];
-    T frac[3];
+__device__ __forceinline__ void cic_axis(T x, T h, int64_t &base, T &frac) {
+    const T s = x / h - (T)0.5;
+    const T i0 = floor(s);
+    base = (int64_t)i0;
+    frac = s - i0;
+}
+
+template <typename T>
+__device__ __forceinline__ void cic_corners(const int64_t base[3], const T frac[3], int n, Stencil<T> &st) {
+    int64_t ix[2], iy[2], iz[2];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const T s = xs[k] / h - (T)0.5;
-        const T i0 = floor(s);
-        base[k] = (int64_t)i0;
-        frac[k] = s - i0;
+    for (int d = 0; d < 2; ++d) {
+        ix[d] = wrap_cell(base[0] + d, n);
+        iy[d] = wrap_cell(base[1] + d, n);
+        iz[d] = wrap_cell(base[2] + d, n);
     }
     int c = 0;
 #pragma unroll
@@ -47,16 +62,21 @@ __device__ __forceinline__ void cic_stencil(T x0, T x1, T x2, int n, T box, Sten
                 const T wx = dx == 0 ? (T)1 - frac[0] : frac[0];
                 const T wy = dy == 0 ? (T)1 - frac[1] : frac[1];
                 const T wz = dz == 0 ? (T)1 - frac[2] : frac[2];
-                int64_t ix = (base[0] + dx) % n;
-                int64_t iy = (base[1] + dy) % n;
-                int64_t iz = (base[2] + dz) % n;
-                ix += ix < 0 ? n : 0;
-                iy += iy < 0 ? n : 0;
-                iz += iz < 0 ? n : 0;
-                st.idx[c] = (ix * n + iy) * n + iz;
+                st.idx[c] = (ix[dx] * n + iy[dy]) * n + iz[dz];
                 st.w[c] = (wx * wy) * wz;
                 ++c;
             }
+}
+
+template <typename T>
+__device__ __forceinline__ void cic_stencil(T x0, T x1, T x2, int n, T box, Stencil<T> &st) {
+    const T h = box / (T)n;
+    int64_t base[3];
+    T frac[3];
+    cic_axis<T>(x0, h, base[0], frac[0]);
+    cic_axis<T>(x1, h, base[1], frac[1]);
+    cic_axis<T>(x2, h, base[2], frac[2]);
+    cic_corners<T>(base, frac, n, st);
 }
 
 template <typename T>
@@ -81,103 +101,127 @@ __global__ void __launch_bounds__(256) k_deposit(const double *__restrict__ pos,
 }
 
 // Same quanta as k_deposit, accumulated per workgroup in an LDS tile first. A workgroup takes
-// kDepChunk consecutive particles of the Morton order, which cover a compact block of cells; if
-// that block (plus the +1 CIC corner) fits the tile, the 8 quanta of each particle are added
-// with LDS atomics and each non-zero tile cell is then added to the grid with one global atomic.
-// Chunks whose block does not fit (they straddle a large Morton boundary) add directly. Integer
-// sums commute, so the grid is bit-identical to k_deposit's.
-constexpr int kDepChunk = 1024;
-constexpr int kDepTile = 4096;
+// up to kDepItem particles of one aligned Morton cube of the sorted order (cubes of at most 8
+// cells on a side, so the CIC footprint, with a margin of two cells for the rounding of the
+// keys, always fits the tile; dense cubes are split into several items), adds their 8 quanta
+// with LDS atomics and then adds each non-zero tile cell to the grid with one global atomic. A
+// particle whose stencil lies outside the tile (only possible for a position outside [0, L))
+// adds directly to the grid. Integer sums commute, so the grid is bit-identical to k_deposit's
+// for any order and any split.
+constexpr int kDepTile = 13 * 13 * 13;
+constexpr uint32_t kDepItem = 2048;
+
+// First sorted index of every level-`level` Morton cube (starts[nodes] = np).
+__global__ void k_node_starts(const uint64_t *__restrict__ keys, size_t np, int shift, uint32_t nodes,
+                              uint32_t *__restrict__ starts) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nodes) return;
+    if (k == nodes) {
+        starts[k] = (uint32_t)np;
+        return;
+    }
+    const uint64_t key = (uint64_t)k << shift;
+    size_t lo = 0, hi = np;
+    while (lo < hi) {
+        const size_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    starts[k] = (uint32_t)lo;
+}
+
+// Work items per cube: ceil(count / kDepItem) (entry `nodes` is 0, so its exclusive scan is the total).
+__global__ void k_node_items(const uint32_t *__restrict__ starts, uint32_t nodes, uint32_t *__restrict__ items) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nodes) return;
+    items[k] = k == nodes ? 0u : (starts[k + 1] - starts[k] + kDepItem - 1) / kDepItem;
+}
+
+__device__ __forceinline__ uint32_t morton_axis(uint32_t k, int axis, int level) {
+    uint32_t v = 0;
+    for (int b = 0; b < level; ++b) v |= ((k >> (3 * b + axis)) & 1u) << b;
+    return v;
+}
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_deposit_tiled(const double *__restrict__ pos, size_t np, int n, double box,
-                                                       double mass, unsigned long long *__restrict__ grid) {
+__global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__ pos, const uint32_t *__restrict__ starts,
+                                                      const uint32_t *__restrict__ item_start, uint32_t nodes,
+                                                      int level, int n, double box, double mass,
+                                                      unsigned long long *__restrict__ grid) {
     __shared__ unsigned long long tile[kDepTile];
-    __shared__ int red[6][8];
-    const size_t c0 = (size_t)blockIdx.x * kDepChunk;
+    // Item b belongs to the cube k with item_start[k] <= b < item_start[k + 1].
+    const uint32_t b = blockIdx.x;
+    if (b >= item_start[nodes]) return;
+    uint32_t lo_k = 0, hi_k = nodes;  // last k in [0, nodes) with item_start[k] <= b
+    while (lo_k < hi_k) {
+        const uint32_t mid = (lo_k + hi_k + 1) >> 1;
+        if (item_start[mid] <= b)
+            lo_k = mid;
+        else
+            hi_k = mid - 1;
+    }
+    const uint32_t k = lo_k;
+    const uint32_t p0 = starts[k] + (b - item_start[k]) * kDepItem;
+    const uint32_t p1 = min(starts[k + 1], p0 + kDepItem);
     const int tid = threadIdx.x;
     const T h = (T)box / (T)n;
-    // Pass 1: bounding block of the stencil bases (unwrapped cell indices, base in [-1, n-1]).
-    int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
-    for (int k = tid; k < kDepChunk; k += 256) {
-        const size_t i = c0 + k;
-        if (i >= np) break;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const T sv = (T)pos[3 * i + a] / h - (T)0.5;
-            const int b = (int)floor(sv);
-            mn[a] = min(mn[a], b);
-            mx[a] = max(mx[a], b);
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        for (int d = 32; d >= 1; d >>= 1) {
-            mn[a] = min(mn[a], __shfl_xor(mn[a], d, 64));
-            mx[a] = max(mx[a], __shfl_xor(mx[a], d, 64));
-        }
-    }
-    const int wave = tid >> 6, lane = tid & 63;
-    if (lane == 0)
-        for (int a = 0; a < 3; ++a) {
-            red[a][wave] = mn[a];
-            red[3 + a][wave] = mx[a];
-        }
-    __syncthreads();
+    // Cells of the cube [k_a n / 2^level, (k_a + 1) n / 2^level), widened by two cells below and
+    // one above (plus the +1 stencil corner).
     int lo[3], dim[3];
+#pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int m0 = red[a][0], m1 = red[3 + a][0];
-        for (int wv = 1; wv < 4; ++wv) {
-            m0 = min(m0, red[a][wv]);
-            m1 = max(m1, red[3 + a][wv]);
-        }
-        lo[a] = m0;
-        dim[a] = m1 - m0 + 2;
+        const int64_t ka = morton_axis(k, a, level);
+        lo[a] = (int)((ka * n) >> level) - 2;
+        const int hi = (int)(((ka + 1) * n + (1 << level) - 1) >> level) + 1;
+        dim[a] = hi - lo[a] + 1;
     }
-    const long long cells = (long long)dim[0] * dim[1] * dim[2];
+    const int cells = dim[0] * dim[1] * dim[2];
+    for (int c = tid; c < cells; c += 256) tile[c] = 0ull;
+    __syncthreads();
     const double scale = mass * 0x1p32;
     const unsigned long long total = (unsigned long long)llrint(scale);
-    const bool use_tile = cells <= kDepTile && dim[0] > 0;
-    if (use_tile) {
-        for (int c = tid; c < (int)cells; c += 256) tile[c] = 0ull;
-        __syncthreads();
-    }
-    for (int k = tid; k < kDepChunk; k += 256) {
-        const size_t i = c0 + k;
-        if (i >= np) break;
-        Stencil<T> st;
-        cic_stencil<T>((T)pos[3 * i], (T)pos[3 * i + 1], (T)pos[3 * i + 2], n, (T)box, st);
-        int b[3];
+    for (uint32_t i = p0 + tid; i < p1; i += 256) {
+        int64_t base[3];
+        T frac[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) b[a] = (int)floor((T)pos[3 * i + a] / h - (T)0.5) - lo[a];
+        for (int a = 0; a < 3; ++a) cic_axis<T>((T)pos[3 * (size_t)i + a], h, base[a], frac[a]);
+        bool inside = true;
+        int off[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int64_t o = base[a] - lo[a];
+            inside = inside && o >= 0 && o <= dim[a] - 2;
+            off[a] = (int)o;
+        }
+        Stencil<T> st;
+        if (!inside) cic_corners<T>(base, frac, n, st);
         unsigned long long used = 0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const unsigned long long q =
-                c < 7 ? (unsigned long long)llrint((double)st.w[c] * scale) : total - used;
+            const int dx = (c >> 2) & 1, dy = (c >> 1) & 1, dz = c & 1;
+            const T wx = dx == 0 ? (T)1 - frac[0] : frac[0];
+            const T wy = dy == 0 ? (T)1 - frac[1] : frac[1];
+            const T wz = dz == 0 ? (T)1 - frac[2] : frac[2];
+            const T w = (wx * wy) * wz;
+            const unsigned long long q = c < 7 ? (unsigned long long)llrint((double)w * scale) : total - used;
             used += q;
-            if (use_tile) {
-                const int dx = (c >> 2) & 1, dy = (c >> 1) & 1, dz = c & 1;
-                const int li = ((b[0] + dx) * dim[1] + (b[1] + dy)) * dim[2] + (b[2] + dz);
-                atomicAdd(&tile[li], q);
-            } else {
+            if (inside)
+                atomicAdd(&tile[((off[0] + dx) * dim[1] + (off[1] + dy)) * dim[2] + (off[2] + dz)], q);
+            else
                 atomicAdd(grid + st.idx[c], q);
-            }
         }
     }
-    if (!use_tile) return;
     __syncthreads();
-    for (int c = tid; c < (int)cells; c += 256) {
+    for (int c = tid; c < cells; c += 256) {
         const unsigned long long v = tile[c];
         if (!v) continue;
         const int iz = c % dim[2];
         const int iy = (c / dim[2]) % dim[1];
         const int ix = c / (dim[2] * dim[1]);
-        int gx = (lo[0] + ix) % n, gy = (lo[1] + iy) % n, gz = (lo[2] + iz) % n;
-        gx += gx < 0 ? n : 0;
-        gy += gy < 0 ? n : 0;
-        gz += gz < 0 ? n : 0;
-        atomicAdd(grid + ((size_t)gx * n + gy) * n + gz, v);
+        const int64_t gx = wrap_cell(lo[0] + ix, n), gy = wrap_cell(lo[1] + iy, n), gz = wrap_cell(lo[2] + iz, n);
+        atomicAdd(grid + (gx * n + gy) * n + gz, v);
     }
 }
 
@@ -263,13 +307,25 @@ void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double
     zero_fill(c, c.grid, c.m * sizeof(unsigned long long));
     if (n == 0) return;
     if (sorted) {
-        const unsigned nb = (unsigned)((n + kDepChunk - 1) / kDepChunk);
+        // Morton cubes of at most 8 cells on a side; the sorted keys are those of sort_particles.
+        const uint32_t nodes = 1u << (3 * c.dep_level);
+        const int shift = 3 * (kKeyBits - c.dep_level);
+        uint32_t *items = c.dep_start + nodes + 1;
+        uint32_t *item_start = items + nodes + 1;
+        hipLaunchKernelGGL(k_node_starts, dim3(blocks(nodes + 1, 256)), dim3(256), 0, c.stream, c.keys, n, shift,
+                           nodes, c.dep_start);
+        hipLaunchKernelGGL(k_node_items, dim3(blocks(nodes + 1, 256)), dim3(256), 0, c.stream, c.dep_start, nodes,
+                           items);
+        size_t bytes = c.dep_scan_bytes;
+        TPM_HIP(rocprim_exclusive_scan_u32(c.dep_scan_tmp, bytes, items, item_start, nodes + 1, c.stream));
+        // At most one partial item per cube plus n / kDepItem full ones.
+        const size_t grid_items = (size_t)nodes + n / kDepItem + 1;
         if (f32_weights)
-            hipLaunchKernelGGL(k_deposit_tiled<float>, dim3(nb), dim3(256), 0, c.stream, pos_src, n, c.n,
-                               c.p.box_size, mass, c.grid);
+            hipLaunchKernelGGL(k_deposit_cube<float>, dim3((unsigned)grid_items), dim3(256), 0, c.stream, pos_src,
+                               c.dep_start, item_start, nodes, c.dep_level, c.n, c.p.box_size, mass, c.grid);
         else
-            hipLaunchKernelGGL(k_deposit_tiled<double>, dim3(nb), dim3(256), 0, c.stream, pos_src, n, c.n,
-                               c.p.box_size, mass, c.grid);
+            hipLaunchKernelGGL(k_deposit_cube<double>, dim3((unsigned)grid_items), dim3(256), 0, c.stream, pos_src,
+                               c.dep_start, item_start, nodes, c.dep_level, c.n, c.p.box_size, mass, c.grid);
         TPM_HIP(hipGetLastError());
         return;
     }

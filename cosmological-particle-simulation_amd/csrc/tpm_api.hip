@@ -11,9 +11,6 @@
 
 namespace tpm {
 size_t sort_temp_bytes(size_t max_n);
-hipError_t rocprim_exclusive_scan_u32(void *tmp, size_t &bytes, const uint32_t *in, uint32_t *out, size_t n,
-                                      hipStream_t s);
-size_t group_scan_bytes(size_t max_n);
 
 static std::string g_create_error;
 
@@ -59,7 +56,7 @@ static void free_all(Ctx &c) {
     void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.sort_tmp, c.spos,
                     c.counter, c.flag, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
                     c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.tree_scan,
-                    c.ngroups_dev, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp, c.stab,
+                    c.ngroups_dev, c.walk_ctr, c.dep_start, c.dep_scan_tmp, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp, c.stab,
                     c.acc_sr, c.members, c.reject, c.mlp.params, c.mlp.grads, c.mlp.m1, c.mlp.m2};
     for (int l = 0; l < 4; ++l) {
         if (c.mlp.wt16[l]) (void)hipFree(c.mlp.wt16[l]);
@@ -100,6 +97,15 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     c.gscan_bytes = group_scan_bytes(mn);
     TPM_HIP(hipMalloc(&c.gscan_tmp, c.gscan_bytes ? c.gscan_bytes : 16));
     TPM_HIP(hipMalloc(&c.ngroups_dev, 16));
+    TPM_HIP(hipMalloc(&c.walk_ctr, 16 * sizeof(uint32_t)));
+    c.dep_level = 0;
+    while (((c.n + (1 << c.dep_level) - 1) >> c.dep_level) > 8) ++c.dep_level;
+    {
+        const size_t nodes1 = (size_t(1) << (3 * c.dep_level)) + 1;
+        TPM_HIP(hipMalloc(&c.dep_start, 3 * nodes1 * sizeof(uint32_t)));
+        TPM_HIP(tpm::rocprim_exclusive_scan_u32(nullptr, c.dep_scan_bytes, nullptr, nullptr, nodes1, nullptr));
+        TPM_HIP(hipMalloc(&c.dep_scan_tmp, c.dep_scan_bytes ? c.dep_scan_bytes : 16));
+    }
     TPM_HIP(hipMalloc(&c.counter, 8 * sizeof(unsigned long long)));
     TPM_HIP(hipMalloc(&c.flag, sizeof(int) * 4));
     TPM_HIP(hipMalloc(&c.grid, c.m * sizeof(unsigned long long)));

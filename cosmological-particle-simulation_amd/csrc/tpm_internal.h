@@ -148,6 +148,11 @@ struct Ctx {
     uint32_t *tree_scan = nullptr;  // per-level child counts, offsets, scan partials, level table
     uint32_t *tree_info = nullptr;  // [0] node count, [1] capacity exceeded
     uint32_t *ngroups_dev = nullptr;  // walk group count (device)
+    uint32_t *walk_ctr = nullptr;     // walk group counters, one per XCD range (16 words)
+    uint32_t *dep_start = nullptr;    // per deposit cube: first sorted index, item count, first item (3 (8^dep_level + 1))
+    void *dep_scan_tmp = nullptr;
+    size_t dep_scan_bytes = 0;
+    int dep_level = 0;                // Morton level of the deposit cubes (at most 8 cells a side)
 };
 
 // ---------------------------------------------------------------------------------------
@@ -184,6 +189,9 @@ void build_tree(Ctx &c, size_t n);
 // c.group_level (sets c.gstart, c.ngroups).
 void build_groups(Ctx &c, size_t n, const uint64_t *keys = nullptr);
 size_t group_scan_bytes(size_t max_n);
+// Exclusive sum of n u32 values (rocPRIM); tmp = nullptr queries the temporary size.
+hipError_t rocprim_exclusive_scan_u32(void *tmp, size_t &bytes, const uint32_t *in, uint32_t *out, size_t n,
+                                      hipStream_t s);
 
 // S(u) table of the group walk for the context's r_split and r_cut.
 void make_short_range_table(Ctx &c);

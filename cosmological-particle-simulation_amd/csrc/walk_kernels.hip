@@ -502,8 +502,14 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
     }
 }
 
-// Persistent workgroups of kWaves wavefronts; each wavefront takes the next group from a global
-// counter until all groups are done (every wavefront reaches the exit test).
+// Persistent workgroups of kWaves wavefronts. The groups (Morton order) are split into
+// kSlots contiguous ranges, one per XCD: workgroups are dealt to the XCDs round-robin, so
+// workgroup b starts on range b mod kSlots and its XCD's L2 keeps serving neighbouring groups,
+// whose source regions overlap. A wavefront takes the next group of its range from that range's
+// counter; when the range is exhausted it moves on to the next one, so every wavefront reaches
+// the exit after visiting each range once and the load stays balanced.
+constexpr int kSlots = 8;
+
 __global__ void __launch_bounds__(64 * kWaves) k_group_walk(WalkArgs w) {
     __shared__ BlockShared bs;
     // The table holds -K S(u) for this evaluation's coupling K.
@@ -514,12 +520,20 @@ __global__ void __launch_bounds__(64 * kWaves) k_group_walk(WalkArgs w) {
     __syncthreads();
     WaveShared &sh = bs.wave[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
-    while (true) {
+    const uint32_t ng = *w.ngroups;
+    int slot = (int)(blockIdx.x % kSlots);
+    for (int visited = 0; visited < kSlots;) {
+        const uint32_t r0 = (uint32_t)(((uint64_t)ng * slot) / kSlots);
+        const uint32_t r1 = (uint32_t)(((uint64_t)ng * (slot + 1)) / kSlots);
         uint32_t g = 0;
-        if (lane == 0) g = atomicAdd(w.next_group, 1u);
+        if (lane == 0) g = atomicAdd(w.next_group + slot, 1u);
         g = __shfl(g, 0, 64);
-        if (g >= *w.ngroups) break;
-        walk_group(sh, bs.stab, w, g);
+        if (g >= r1 - r0) {
+            slot = (slot + 1) % kSlots;
+            ++visited;
+            continue;
+        }
+        walk_group(sh, bs.stab, w, r0 + g);
     }
 }
 
@@ -685,8 +699,8 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     w.gstart = c.gstart;
     w.members = members;
     w.ngroups = c.ngroups_dev;
-    w.next_group = reinterpret_cast<uint32_t *>(c.flag) + 2;
-    zero_fill(c, w.next_group, sizeof(uint32_t));
+    w.next_group = c.walk_ctr;
+    zero_fill(c, w.next_group, kSlots * sizeof(uint32_t));
     if (c.walk_blocks == 0) {
         int per_cu = 0;
         TPM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_walk, 64 * kWaves, 0));
