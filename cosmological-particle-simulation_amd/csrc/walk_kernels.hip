@@ -50,7 +50,11 @@ struct WalkArgs {
     float eps2;
     float inv2rs;
     float negk;
-    float tab_scale;  // (1 / (2 r_s)) / (u_cut / kSTab): sqrt(r^2) * tab_scale indexes the S table
+    // The S table is indexed by x = sqrt(r^2 + eps^2) * c (c = kSTab / rho_max): s' = r^2 c^2 + eps^2 c^2
+    // gives x = s' rsqrt(s') and 1 / (r^2 + eps^2)^{3/2} = c^3 rsqrt(s')^3, so the table carries -K c^3.
+    float sc2;   // c^2
+    float e2s;   // eps^2 c^2
+    float tabk;  // -K c^3
     const float2 *stab;
     uint32_t *next_group;
     const uint32_t *ngroups;  // device word written by the group builder
@@ -161,17 +165,19 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Pair force with -K S(u) interpolated from the LDS table (linear in u over kSTab intervals of
-// [0, u_cut (1 + 1e-4)]; interpolation error below 1e-6). Used for particle pairs, which satisfy
-// r <= r_cut, so the interval index is at most kSTab - 1 and needs no clamp.
+// Pair force with -K c^3 S interpolated from the LDS table, linear in rho = sqrt(r^2 + eps^2) over
+// kSTab intervals of [0, rho_max], rho_max = sqrt((r_cut (1 + 1e-4))^2 + eps^2) (relative
+// interpolation error of S below 1e-5). One reciprocal square root gives both the table index
+// and the softened 1 / (r^2 + eps^2)^{3/2}. Used for particle pairs, which satisfy r <= r_cut, so
+// the interval index is at most kSTab - 1 and needs no clamp.
 __device__ __forceinline__ void pair_tab(float dx, float dy, float dz, float r2, const float2 *stab,
                                          const WalkArgs &w, float &ax, float &ay, float &az) {
-    const float s = r2 + w.eps2;
+    const float s = fmaf(r2, w.sc2, w.e2s);
     const float rs = __builtin_amdgcn_rsqf(s);
-    const float inv3 = rs * rs * rs;
-    const float x = __builtin_amdgcn_sqrtf(r2) * w.tab_scale;
+    const float x = s * rs;
     const float fr = __builtin_amdgcn_fractf(x);
     const float2 t = stab[(int)x];
+    const float inv3 = rs * rs * rs;
     const float f = fmaf(fr, t.y, t.x) * inv3;
     ax = fmaf(dx, f, ax);
     ay = fmaf(dy, f, ay);
@@ -515,7 +521,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_group_walk(WalkArgs w) {
     // The table holds -K S(u) for this evaluation's coupling K.
     for (int i = threadIdx.x; i <= kSTab; i += blockDim.x) {
         const float2 t = w.stab[i];
-        bs.stab[i] = make_float2(w.negk * t.x, w.negk * t.y);
+        bs.stab[i] = make_float2(w.tabk * t.x, w.tabk * t.y);
     }
     __syncthreads();
     WaveShared &sh = bs.wave[threadIdx.x >> 6];
@@ -683,7 +689,10 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     w.eps2 = (float)c.p.softening * (float)c.p.softening;
     w.inv2rs = (float)(1.0 / (2.0 * c.p.r_split));
     w.negk = (float)(-k);
-    w.tab_scale = (float)(c.stab_scale);
+    const double sc = c.stab_scale;
+    w.sc2 = (float)(sc * sc);
+    w.e2s = (float)(c.p.softening * c.p.softening * sc * sc);
+    w.tabk = (float)(-k * sc * sc * sc);
     w.stab = c.stab;
     w.acc = d_acc;
     w.counter = c.counter;
@@ -714,23 +723,29 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     TPM_HIP(hipGetLastError());
 }
 
-// S(u) = erfc(u) + (2/sqrt(pi)) u exp(-u^2) (src/tree.rs:54-57) tabulated in FP64 on
-// [0, u_cut (1 + 1e-4)], u_cut = r_cut / (2 r_s): kSTab + 1 points, value and slope per interval.
+// S(u) = erfc(u) + (2/sqrt(pi)) u exp(-u^2) (src/tree.rs:54-57), u = r / (2 r_s), tabulated in FP64
+// against rho = sqrt(r^2 + eps^2) on [0, rho_max], rho_max = sqrt((r_cut (1 + 1e-4))^2 + eps^2):
+// kSTab + 1 points, value and slope per interval (entries below rho = eps, never read, hold S(0)).
 void make_short_range_table(Ctx &c) {
     if (!(c.p.r_split > 0) || !(c.p.r_cut > 0)) {
         c.stab_scale = 0.0;
         return;
     }
-    const double umax = c.p.r_cut / (2.0 * c.p.r_split) * (1.0 + 1e-4);
-    const double du = umax / kSTab;
-    auto S = [](double u) { return std::erfc(u) + 1.1283791670955126 * u * std::exp(-u * u); };
+    const double eps2 = c.p.softening * c.p.softening;
+    const double rc = c.p.r_cut * (1.0 + 1e-4);
+    const double rho_max = std::sqrt(rc * rc + eps2);
+    const double drho = rho_max / kSTab;
+    auto S = [&](double rho) {
+        const double u = std::sqrt(std::max(rho * rho - eps2, 0.0)) / (2.0 * c.p.r_split);
+        return std::erfc(u) + 1.1283791670955126 * u * std::exp(-u * u);
+    };
     std::vector<float2> t(kSTab + 1);
     for (int i = 0; i <= kSTab; ++i) {
-        const double v0 = S(i * du), v1 = S((i + 1) * du);
+        const double v0 = S(i * drho), v1 = S((i + 1) * drho);
         t[i] = make_float2((float)v0, (float)(v1 - v0));
     }
     TPM_HIP(hipMemcpy(c.stab, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
-    c.stab_scale = 1.0 / (2.0 * c.p.r_split) / du;
+    c.stab_scale = 1.0 / drho;
 }
 
 void walk_judge_f64(Ctx &c, size_t n, const uint32_t *d_targets_sorted, size_t m, double k, double theta,
