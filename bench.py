@@ -59,6 +59,19 @@ def cpu_baseline(n_side=128, theta=0.35, threads=None):
                       f"mesh = particle grid), {dt:.2f} s wall"}
 
 
+def walk_traffic(n, workload, tuning):
+    """HBM bytes per k_group_walk launch from the committed rocprofv3 counter summary of this
+    workload (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a copy of known
+    size), or None when this run is not that workload."""
+    path = os.path.join(REPO, "profiles", "k_group_walk_pmc_zeldovich_512cubed.md")
+    if n != 512 or workload != "zeldovich" or tuning is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def judge_error(solver, pos, acc, npart, a, m=16384, seed=12345):
     """RMS / p99 relative error of acc (device) against the FP64 judge on a seeded subsample."""
     rng = np.random.default_rng(seed)
@@ -251,8 +264,9 @@ def main():
     stages = {k: round(mean(k), 3) for k in ("ms_sort", "ms_deposit", "ms_fft", "ms_interp", "ms_tree", "ms_walk",
                                                "ms_total")}
     inter = mean("interactions")
-    walk_s = mean("ms_walk") / 1000.0
+    walk_s = mean("ms_walk_kernel") / 1000.0  # the k_group_walk launch alone (HIP events on its stream)
     achieved = inter * FLOP_PER_INTERACTION / walk_s / 1e12 if walk_s > 0 else 0.0
+    traffic = walk_traffic(n, args.workload, tuning)
 
     result = {
         "metric": "TreePM particle-force evals/s at 512^3 + RMS force err vs FP64; learned vs plain",
@@ -278,7 +292,8 @@ def main():
                                    "kept": mean("kept") / max(mean("groups"), 1)},
         "roofline": {"bound": "mfma", "kernel": "k_group_walk (FP32 VALU pair arithmetic)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "walk_kernel_ms": round(mean("ms_walk_kernel"), 3)},
     }
 
     if not args.no_judge:

@@ -24,6 +24,14 @@ struct Stencil {
     T w[8];
 };
 
+// Workgroups are dealt to the 8 XCDs round-robin; this renumbering gives each XCD a contiguous
+// range of logical blocks, so blocks of neighbouring particles (Morton order) share its L2. It is
+// a bijection of [0, g).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t g) {
+    const uint32_t q = g / 8, r = g % 8, x = b % 8;
+    return x * q + min(x, r) + b / 8;
+}
+
 // Periodic wrap of an unwrapped cell index (rem_euclid). For positions in [0, L) the stencil
 // corners lie in [-1, n], so one compare-and-add covers them; anything else takes the modulo.
 __device__ __forceinline__ int64_t wrap_cell(int64_t v, int n) {
@@ -152,7 +160,7 @@ __global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__
                                                       unsigned long long *__restrict__ grid) {
     __shared__ unsigned long long tile[kDepTile];
     // Item b belongs to the cube k with item_start[k] <= b < item_start[k + 1].
-    const uint32_t b = blockIdx.x;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     if (b >= item_start[nodes]) return;
     uint32_t lo_k = 0, hi_k = nodes;  // last k in [0, nodes) with item_start[k] <= b
     while (lo_k < hi_k) {
@@ -240,7 +248,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_interp3(const double *__restrict__ pos, const uint32_t *__restrict__ order,
                                                  size_t np, int n, double box, const float *__restrict__ g, size_t m,
                                                  float *__restrict__ acc) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t i = (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= np) return;
     Stencil<T> st;
     cic_stencil<T>((T)pos[3 * i], (T)pos[3 * i + 1], (T)pos[3 * i + 2], n, (T)box, st);

@@ -53,6 +53,8 @@ static void validate(const tpm_params &p) {
 }
 
 static void free_all(Ctx &c) {
+    for (hipEvent_t &e : c.walk_ev)
+        if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.sort_tmp, c.spos,
                     c.counter, c.flag, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
                     c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.tree_scan,
@@ -207,6 +209,12 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
         c.stats.ms_tree = tm.ms(4, 5);
         c.stats.ms_walk = tm.ms(5, 6);
         c.stats.ms_total = tm.ms(0, 6);
+        c.stats.ms_walk_kernel = 0.0;
+        if (treepm && n > 0 && c.walk_ev[1]) {
+            float t = 0.f;
+            TPM_HIP(hipEventElapsedTime(&t, c.walk_ev[0], c.walk_ev[1]));
+            c.stats.ms_walk_kernel = t;
+        }
         unsigned long long cnt[4] = {0, 0, 0, 0};
         TPM_HIP(hipMemcpy(cnt, c.counter, sizeof(cnt), hipMemcpyDeviceToHost));
         c.stats.interactions = cnt[0];
@@ -765,6 +773,12 @@ int tpm_accelerations_learned_dev(tpm_ctx *ctx, const double *d_pos, size_t n, d
             c.stats.ms_interp = tm.ms(1, 2);
             c.stats.ms_walk = tm.ms(2, 3);
             c.stats.ms_total = tm.ms(0, 3);
+            c.stats.ms_walk_kernel = 0.0;
+            if (nrej > 0 && c.walk_ev[1]) {
+                float t = 0.f;
+                TPM_HIP(hipEventElapsedTime(&t, c.walk_ev[0], c.walk_ev[1]));
+                c.stats.ms_walk_kernel = t;
+            }
         }
         return TPM_OK;
     });

@@ -137,6 +137,7 @@ struct Ctx {
     double stab_scale = 0.0;
     int num_cu = 0;
     unsigned walk_blocks = 0;
+    hipEvent_t walk_ev[2] = {nullptr, nullptr};  // around the walk kernel when timing
     // Walk groups: particles [gstart[g], gstart[g+1]) of the Morton order.
     uint32_t *gstart = nullptr;     // max_n + 1
     uint32_t *gscratch = nullptr;   // 2 max_n words

@@ -719,8 +719,14 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     // a bound on the group count needs.
     const size_t bound = ((members ? count : n) + kWaves - 1) / kWaves;
     const unsigned nblk = (unsigned)std::max<size_t>(1, std::min<size_t>(c.walk_blocks, bound));
+    if (c.timing) {
+        if (!c.walk_ev[0]) TPM_HIP(hipEventCreate(&c.walk_ev[0]));
+        if (!c.walk_ev[1]) TPM_HIP(hipEventCreate(&c.walk_ev[1]));
+        TPM_HIP(hipEventRecord(c.walk_ev[0], c.stream));
+    }
     hipLaunchKernelGGL(k_group_walk, dim3(nblk), dim3(64 * kWaves), 0, c.stream, w);
     TPM_HIP(hipGetLastError());
+    if (c.timing) TPM_HIP(hipEventRecord(c.walk_ev[1], c.stream));
 }
 
 // S(u) = erfc(u) + (2/sqrt(pi)) u exp(-u^2) (src/tree.rs:54-57), u = r / (2 r_s), tabulated in FP64

@@ -112,6 +112,7 @@ class Stats(ctypes.Structure):
         ("groups", ctypes.c_uint64),
         ("listed", ctypes.c_uint64),
         ("kept", ctypes.c_uint64),
+        ("ms_walk_kernel", ctypes.c_double),
     ]
 
     def as_dict(self):

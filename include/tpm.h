@@ -75,6 +75,7 @@ typedef struct tpm_stats {
     uint64_t groups;       /* walk groups (wavefronts) */
     uint64_t listed;       /* particle sources listed by the group walks (before the box test) */
     uint64_t kept;         /* particle sources kept for evaluation (each costs one 64-lane step) */
+    double ms_walk_kernel; /* the walk kernel alone (ms_walk also builds the walk groups) */
 } tpm_stats;
 
 /* Version string of the library. */
