@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "libtpm.so")
 BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["tpm_api.hip", "mesh_kernels.hip", "fft_kernels.hip", "tree_kernels.hip", "walk_kernels.hip", "ic_kernels.hip", "kdk_kernels.hip", "mlp_kernels.hip", "ensemble.hip"]
+SOURCES = ["tpm_api.hip", "mesh_kernels.hip", "fft_kernels.hip", "tree_kernels.hip", "walk_kernels.hip", "ic_kernels.hip", "kdk_kernels.hip", "mlp_kernels.hip", "ensemble.hip", "runner.hip"]
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
     # Packed FP32 instructions stay off: the compiler would otherwise form v_pk_* from float code.
@@ -48,6 +48,14 @@ def build(jobs=None, verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, OUT)
+    # The headless runner (a small C++ program over the C ABI), next to the library.
+    exe = os.path.join(HERE, "tpm_headless")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(REPO, "include"), os.path.join(CSRC, "headless_main.cpp"),
+           "-L", HERE, "-ltpm", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building tpm_headless failed:\n{r.stderr}")
+    os.replace(exe + ".tmp", exe)
     if verbose:
         print(f"built {OUT}")
     return OUT

@@ -279,14 +279,25 @@ __global__ void __launch_bounds__(256) k_fft_y(typename Cx<R>::T *spec, typename
 //   mode 1, Zel'dovich displacement: G = -c0 sqrt(k^-2 exp(-|k| / c1)) / k^2, the power law
 //           P(k) = k^-2 exp(-k / k_c) with c1 = k_c and c0 = sqrt(n^3 / V) / n^3 (the amplitude is
 //           rescaled afterwards to a target displacement RMS).
+//   mode 2, Zel'dovich displacement with the BBKS spectrum (src/ics.rs:20-33, src/cosmology.rs:93-103):
+//           G = -c0 sqrt(k^n_s) T(k) / k^2, c1 = n_s, c2 = Gamma = Omega_m h, c0 including sqrt(amplitude).
 struct GreenArgs {
     int mode;
-    double c0, c1;
+    double c0, c1, c2;
 };
+
+__device__ __forceinline__ double bbks_transfer(double k, double gamma) {
+    const double q = k / gamma;
+    if (q == 0.0) return 1.0;
+    const double a = 16.1 * q, b = 5.46 * q, c = 6.71 * q;
+    const double poly = 1.0 + 3.89 * q + a * a + b * b * b + (c * c) * (c * c);
+    return log1p(2.34 * q) / (2.34 * q) * pow(poly, -0.25);
+}
 
 __device__ __forceinline__ double green_factor(const GreenArgs &g, double k2) {
     if (g.mode == 0) return -g.c0 / k2 * exp(-k2 * g.c1);
     const double k = sqrt(k2);
+    if (g.mode == 2) return -g.c0 * pow(k, 0.5 * g.c1) * bbks_transfer(k, g.c2) / k2;
     return -g.c0 * sqrt(exp(-k / g.c1) / k2) / k2;
 }
 
@@ -551,6 +562,7 @@ static GreenArgs gravity(const Ctx &c, double a, double r_split) {
     g.mode = 0;
     g.c0 = 1.5 * c.p.omega_m / a / ((double)c.n * c.n * c.n);
     g.c1 = r_split * r_split;
+    g.c2 = 0.0;
     return g;
 }
 
@@ -590,7 +602,7 @@ void pm_from_grid_f64(Ctx &c, size_t n_particles, double a, double r_split) {
 
 // Zel'dovich displacement grids from a white-noise grid on the device, through the FP64
 // pipeline. Writes c.dforce = (psi_x, psi_y, psi_z) on the mesh points.
-void displacement_from_noise(Ctx &c, const float *d_noise, double kc) {
+void displacement_from_noise(Ctx &c, const float *d_noise, const DisplacementSpec &spec) {
     if (!c.dspecA) {
         TPM_HIP(hipMalloc(&c.dspecA, c.hs * sizeof(double2)));
         TPM_HIP(hipMalloc(&c.dspecB, c.hs * sizeof(double2)));
@@ -602,11 +614,19 @@ void displacement_from_noise(Ctx &c, const float *d_noise, double kc) {
     src.delta = d_noise;
     src.delta_out = nullptr;
     GreenArgs g;
-    g.mode = 1;
     const double m = (double)c.n * c.n * c.n;
     const double vol = c.p.box_size * c.p.box_size * c.p.box_size;
     g.c0 = sqrt(m / vol) / m;
-    g.c1 = kc;
+    if (spec.bbks) {
+        g.mode = 2;
+        g.c0 *= sqrt(spec.amplitude);
+        g.c1 = spec.n_s;
+        g.c2 = spec.gamma;
+    } else {
+        g.mode = 1;
+        g.c1 = spec.k_c;
+        g.c2 = 0.0;
+    }
     pm_pipeline<double>(c, src, c.dspecA, c.dspecB, c.dspecC, c.tw64, c.dforce, g);
 }
 

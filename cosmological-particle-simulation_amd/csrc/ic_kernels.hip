@@ -78,13 +78,87 @@ __global__ void k_place(const double *__restrict__ g, int n, double box, double 
 
 static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// ---------------------------------------------------------------- BBKS / sigma_8 spectrum
+
+// BBKS transfer function, dark matter only (src/cosmology.rs:93-103): q = k / (Omega_m h).
+static double bbks_transfer_host(double k, double gamma) {
+    const double q = k / gamma;
+    if (q == 0.0) return 1.0;
+    const double a = 16.1 * q, b = 5.46 * q, c = 6.71 * q;
+    const double poly = 1.0 + 3.89 * q + a * a + b * b * b + (c * c) * (c * c);
+    return std::log1p(2.34 * q) / (2.34 * q) * std::pow(poly, -0.25);
+}
+
+// Spherical top-hat window W(x) = 3 (sin x - x cos x) / x^3, 1 below x = 1e-4 (src/ics.rs:67-74).
+static double tophat(double x) {
+    if (x < 1e-4) return 1.0;
+    return 3.0 * (std::sin(x) - x * std::cos(x)) / (x * x * x);
+}
+
+double PowerSpec::power(double k) const {
+    if (k == 0.0) return 0.0;
+    const double t = bbks_transfer_host(k, gamma);
+    return amplitude * std::pow(k, n_s) * t * t;
+}
+
+// sigma(R)^2 = (1 / 2 pi^2) int dk k^2 P(k) W(kR)^2, Simpson's rule over ln k in [1e-4, 1e3] with
+// 8192 intervals (src/ics.rs:35-52), in FP64.
+double PowerSpec::sigma(double radius) const {
+    const int steps = 8192;
+    const double lo = std::log(1e-4), hi = std::log(1e3);
+    const double h = (hi - lo) / steps;
+    auto f = [&](double lk) {
+        const double k = std::exp(lk);
+        const double w = tophat(k * radius);
+        return k * k * k * power(k) * w * w;
+    };
+    double sum = f(lo) + f(hi);
+    for (int i = 1; i < steps; ++i) sum += (i % 2 == 1 ? 4.0 : 2.0) * f(lo + i * h);
+    const double integral = sum * h / 3.0;
+    return std::sqrt(integral / (2.0 * M_PI * M_PI));
+}
+
+// PowerSpectrum::normalized (src/ics.rs:54-64): the amplitude that gives sigma(radius) = sigma8.
+PowerSpec normalized_power(const tpm_cosmology &cosmo, double h0, double n_s, double sigma8, double radius) {
+    PowerSpec p;
+    p.amplitude = 1.0;
+    p.n_s = n_s;
+    p.gamma = cosmo.omega_m * h0 / 100.0;
+    const double s = p.sigma(radius);
+    p.amplitude = (sigma8 / s) * (sigma8 / s);
+    return p;
+}
+
+void zeldovich_bbks(Ctx &c, const tpm_cosmology &cosmo, const PowerSpec &ps, uint64_t seed, double a_init,
+                    double *d_pos, double *d_mom) {
+    TPM_REQUIRE(c.m <= c.max_n, "context capacity must hold mesh_n^3 particles");
+    TPM_REQUIRE(a_init > 0 && std::isfinite(a_init), "a_init must be positive");
+    hipLaunchKernelGGL(k_noise, dim3(blocks(c.m, 256)), dim3(256), 0, c.stream, c.delta, c.m, seed);
+    TPM_HIP(hipGetLastError());
+    DisplacementSpec spec;
+    spec.bbks = true;
+    spec.amplitude = ps.amplitude;
+    spec.n_s = ps.n_s;
+    spec.gamma = ps.gamma;
+    displacement_from_noise(c, c.delta, spec);
+    // x = q + D Psi, p = a^2 f E D Psi (src/ics.rs:155-177): the displacement at a_init times a^2 f E.
+    double d = 0.0, f = 0.0;
+    growth(cosmo, a_init, &d, &f);
+    const double e = hubble_e(cosmo, a_init);
+    hipLaunchKernelGGL(k_place, dim3(blocks(c.m, 256)), dim3(256), 0, c.stream, c.dforce, c.n, c.p.box_size, d,
+                       a_init * a_init * f * e, d_pos, d_mom);
+    TPM_HIP(hipGetLastError());
+}
+
 void zeldovich(Ctx &c, uint64_t seed, double kc, double disp_rms, double mom_factor, double *d_pos, double *d_mom) {
     TPM_REQUIRE(c.m <= c.max_n, "context capacity must hold mesh_n^3 particles");
     TPM_REQUIRE(kc > 0 && disp_rms >= 0, "k_c must be positive and the displacement RMS >= 0");
     // Noise lives in the delta grid buffer (n^3 floats).
     hipLaunchKernelGGL(k_noise, dim3(blocks(c.m, 256)), dim3(256), 0, c.stream, c.delta, c.m, seed);
     TPM_HIP(hipGetLastError());
-    displacement_from_noise(c, c.delta, kc);
+    DisplacementSpec spec;
+    spec.k_c = kc;
+    displacement_from_noise(c, c.delta, spec);
     const unsigned nb = 1024;
     double *part = reinterpret_cast<double *>(c.judge_buf);
     if (c.judge_cap < nb) {

@@ -294,6 +294,59 @@ __global__ void k_interp_judge(const double *__restrict__ pos, const uint32_t *_
     }
 }
 
+// Per-step outputs (src/runner.rs:262-284 snapshot, src/frame.rs:56-64 rsd_shift) for FP64 state:
+// positions rounded to float, speed |p| / a, density max(1 + delta(x), 0) by the FP64 stencil with
+// weights rounded to float and summed in float (the FP64 interpolation convention), redshift-space
+// shift p / (a^2 E). Outputs may be null.
+__global__ void k_snapshot(const double *__restrict__ pos, const double *__restrict__ mom, size_t np, double a,
+                           double inv_a2e, int n, double box, const float *__restrict__ delta,
+                           float *__restrict__ opos, float *__restrict__ ospeed, float *__restrict__ odens,
+                           float *__restrict__ orsd) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const double x0 = pos[3 * i], x1 = pos[3 * i + 1], x2 = pos[3 * i + 2];
+    if (opos) {
+        opos[3 * i] = (float)x0;
+        opos[3 * i + 1] = (float)x1;
+        opos[3 * i + 2] = (float)x2;
+    }
+    if (ospeed || orsd) {
+        const double p0 = mom[3 * i], p1 = mom[3 * i + 1], p2 = mom[3 * i + 2];
+        if (ospeed) ospeed[i] = (float)(sqrt(p0 * p0 + p1 * p1 + p2 * p2) / a);
+        if (orsd) {
+            orsd[3 * i] = (float)(p0 * inv_a2e);
+            orsd[3 * i + 1] = (float)(p1 * inv_a2e);
+            orsd[3 * i + 2] = (float)(p2 * inv_a2e);
+        }
+    }
+    if (odens) {
+        Stencil<double> st;
+        cic_stencil<double>(x0, x1, x2, n, box, st);
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s = s + delta[st.idx[c]] * (float)st.w[c];
+        odens[i] = fmaxf(1.0f + s, 0.0f);
+    }
+}
+
+// src/frame.rs:33-52 downsample_grid: texel (x, y, z) of the m^3 output (x fastest) is 1 + the mean
+// of the f^3 mesh block at (x f, y f, z f), summed in the reference's loop order in float.
+__global__ void k_downsample(const float *__restrict__ delta, int n, int f, int m, float norm,
+                             float *__restrict__ out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t mm = (size_t)m * m * m;
+    if (t >= mm) return;
+    const int x = (int)(t % m), y = (int)((t / m) % m), z = (int)(t / ((size_t)m * m));
+    float sum = 0.0f;
+    for (int dx = 0; dx < f; ++dx)
+        for (int dy = 0; dy < f; ++dy)
+            for (int dz = 0; dz < f; ++dz) {
+                const size_t ix = (size_t)x * f + dx, iy = (size_t)y * f + dy, iz = (size_t)z * f + dz;
+                sum += delta[(ix * n + iy) * n + iz];
+            }
+    out[t] = 1.0f + sum * norm;
+}
+
 __global__ void k_zero32(uint32_t *__restrict__ p, size_t count) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
@@ -379,6 +432,30 @@ void interp_judge_f64(Ctx &c, const uint32_t *d_targets_sorted, size_t m, double
     if (m == 0) return;
     hipLaunchKernelGGL(k_interp_judge, dim3(blocks(m, 256)), dim3(256), 0, c.stream, c.spos, d_targets_sorted, m, c.n,
                        c.p.box_size, c.dforce, c.m, d_out);
+    TPM_HIP(hipGetLastError());
+}
+
+void snapshot(Ctx &c, const double *pos, const double *mom, size_t n, double a, double e, const float *delta,
+              float *opos, float *ospeed, float *odens, float *orsd) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_snapshot, dim3(blocks(n, 256)), dim3(256), 0, c.stream, pos, mom, n, a, 1.0 / (a * a * e),
+                       c.n, c.p.box_size, delta, opos, ospeed, odens, orsd);
+    TPM_HIP(hipGetLastError());
+}
+
+int downsample_side(int n, int *f_out) {
+    int f = (n + 127) / 128;
+    if (f < 1) f = 1;
+    if (f_out) *f_out = f;
+    return n / f;
+}
+
+void downsample_grid(Ctx &c, const float *delta, float *out) {
+    int f = 1;
+    const int m = downsample_side(c.n, &f);
+    const float norm = 1.0f / (float)(f * f * f);
+    const size_t mm = (size_t)m * m * m;
+    hipLaunchKernelGGL(k_downsample, dim3(blocks(mm, 256)), dim3(256), 0, c.stream, delta, c.n, f, m, norm, out);
     TPM_HIP(hipGetLastError());
 }
 

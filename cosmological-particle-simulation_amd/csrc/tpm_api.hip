@@ -794,6 +794,103 @@ int tpm_kdk_factors(const tpm_cosmology *cosmo, double a0, double a1, double *am
     return TPM_OK;
 }
 
+int tpm_snapshot_dev(tpm_ctx *ctx, const double *d_pos, const double *d_mom, size_t n, double a, double e,
+                     const float *d_delta, float *d_pos32, float *d_speed, float *d_density, float *d_rsd,
+                     void *stream) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(a > 0 && e > 0 && std::isfinite(a) && std::isfinite(e), "a and E must be positive");
+        TPM_REQUIRE(n == 0 || d_pos != nullptr, "null positions");
+        TPM_REQUIRE(!(d_speed || d_rsd) || d_mom != nullptr, "speeds and shifts need momenta");
+        TPM_REQUIRE(!d_density || d_delta != nullptr, "densities need the density-contrast mesh");
+        hipStream_t saved = c.stream;
+        if (stream) c.stream = (hipStream_t)stream;
+        try {
+            tpm::snapshot(c, d_pos, d_mom, n, a, e, d_delta, d_pos32, d_speed, d_density, d_rsd);
+        } catch (...) {
+            c.stream = saved;
+            throw;
+        }
+        c.stream = saved;
+        return TPM_OK;
+    });
+}
+
+int tpm_downsample_grid_dev(tpm_ctx *ctx, const float *d_delta, float *d_out, int32_t *side, void *stream) {
+    return guarded(ctx, [&](Ctx &c) {
+        const int m = tpm::downsample_side(c.n, nullptr);
+        if (side) *side = m;
+        if (!d_out) return TPM_OK;
+        TPM_REQUIRE(d_delta != nullptr, "null density-contrast mesh");
+        hipStream_t saved = c.stream;
+        if (stream) c.stream = (hipStream_t)stream;
+        try {
+            tpm::downsample_grid(c, d_delta, d_out);
+        } catch (...) {
+            c.stream = saved;
+            throw;
+        }
+        c.stream = saved;
+        return TPM_OK;
+    });
+}
+
+int tpm_power_spectrum(const tpm_cosmology *cosmo, double h0, double n_s, double sigma8, double k, double *p,
+                       double *amplitude) {
+    if (!cosmo || !(h0 > 0) || !(sigma8 > 0) || !(k >= 0) || !std::isfinite(k)) return TPM_E_INVALID;
+    const tpm::PowerSpec ps = tpm::normalized_power(*cosmo, h0, n_s, sigma8, 8.0);
+    if (p) *p = ps.power(k);
+    if (amplitude) *amplitude = ps.amplitude;
+    return TPM_OK;
+}
+
+int tpm_power_sigma(const tpm_cosmology *cosmo, double h0, double n_s, double sigma8, double radius, double *sigma) {
+    if (!cosmo || !(h0 > 0) || !(sigma8 > 0) || !(radius > 0) || !sigma) return TPM_E_INVALID;
+    *sigma = tpm::normalized_power(*cosmo, h0, n_s, sigma8, 8.0).sigma(radius);
+    return TPM_OK;
+}
+
+int tpm_zeldovich_bbks_dev(tpm_ctx *ctx, const tpm_cosmology *cosmo, double h0, double n_s, double sigma8,
+                           uint64_t seed, double a_init, double *d_pos, double *d_mom, void *stream) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(cosmo != nullptr && d_pos != nullptr, "null argument");
+        TPM_REQUIRE(h0 > 0 && sigma8 > 0, "h0 and sigma8 must be positive");
+        const tpm::PowerSpec ps = tpm::normalized_power(*cosmo, h0, n_s, sigma8, 8.0);
+        hipStream_t saved = c.stream;
+        if (stream) c.stream = (hipStream_t)stream;
+        try {
+            tpm::zeldovich_bbks(c, *cosmo, ps, seed, a_init, d_pos, d_mom);
+        } catch (...) {
+            c.stream = saved;
+            throw;
+        }
+        c.stream = saved;
+        return TPM_OK;
+    });
+}
+
+int tpm_zeldovich_bbks(tpm_ctx *ctx, const tpm_cosmology *cosmo, double h0, double n_s, double sigma8, uint64_t seed,
+                       double a_init, double *pos, double *mom) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(cosmo != nullptr && pos != nullptr, "null argument");
+        TPM_REQUIRE(h0 > 0 && sigma8 > 0, "h0 and sigma8 must be positive");
+        TPM_REQUIRE(c.m <= c.max_n, "context capacity must hold mesh_n^3 particles");
+        const tpm::PowerSpec ps = tpm::normalized_power(*cosmo, h0, n_s, sigma8, 8.0);
+        double *d_mom = nullptr;
+        TPM_HIP(hipMalloc(&d_mom, 3 * c.m * sizeof(double)));
+        try {
+            tpm::zeldovich_bbks(c, *cosmo, ps, seed, a_init, c.pos, d_mom);
+            TPM_HIP(hipMemcpyAsync(pos, c.pos, 3 * c.m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            if (mom) TPM_HIP(hipMemcpyAsync(mom, d_mom, 3 * c.m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+            TPM_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            (void)hipFree(d_mom);
+            throw;
+        }
+        (void)hipFree(d_mom);
+        return TPM_OK;
+    });
+}
+
 int tpm_growth(const tpm_cosmology *cosmo, double a, double *d, double *f, double *e) {
     if (!cosmo || !(a > 0) || !std::isfinite(a)) return TPM_E_INVALID;
     tpm::growth(*cosmo, a, d, f);

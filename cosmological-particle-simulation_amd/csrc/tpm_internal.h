@@ -208,6 +208,12 @@ void walk_judge_f64(Ctx &c, size_t n, const uint32_t *d_targets_sorted, size_t m
 // FP64 interpolation of c.dforce at listed sorted indices.
 void interp_judge_f64(Ctx &c, const uint32_t *d_targets_sorted, size_t m, double *d_out);
 
+// Per-step outputs (mesh_kernels.hip): snapshot of FP64 state and the downsampled 1 + delta grid.
+void snapshot(Ctx &c, const double *pos, const double *mom, size_t n, double a, double e, const float *delta,
+              float *opos, float *ospeed, float *odens, float *orsd);
+int downsample_side(int n, int *f_out);
+void downsample_grid(Ctx &c, const float *delta, float *out);
+
 // Force evaluation on device-resident FP64 positions (tpm_api.hip); no host synchronisation
 // unless timing is on.
 void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float *d_acc, float *d_delta);
@@ -223,7 +229,25 @@ void convert_f32_to_f64(Ctx &c, const float *src, double *dst, size_t count);
 void check_finite(Ctx &c, const float *d_acc, size_t count);
 
 // Zel'dovich displacement grids from device white noise (n^3 floats): c.dforce.
-void displacement_from_noise(Ctx &c, const float *d_noise, double kc);
+// Spectrum of the Zel'dovich displacement field: the power law k^-2 exp(-k / k_c) (rescaled to a
+// displacement RMS afterwards) or amplitude k^n_s T_BBKS(k / Gamma)^2 (src/ics.rs:20-33).
+struct DisplacementSpec {
+    bool bbks = false;
+    double k_c = 0.0;
+    double amplitude = 0.0, n_s = 0.0, gamma = 0.0;
+};
+void displacement_from_noise(Ctx &c, const float *d_noise, const DisplacementSpec &spec);
+
+// PowerSpectrum with the BBKS transfer (src/ics.rs:20-64), FP64 on the host.
+struct PowerSpec {
+    double amplitude = 1.0, n_s = 1.0, gamma = 0.2;
+    double power(double k) const;
+    double sigma(double radius) const;
+};
+PowerSpec normalized_power(const tpm_cosmology &cosmo, double h0, double n_s, double sigma8, double radius);
+// Lattice Zel'dovich ICs from the BBKS spectrum at a_init (src/ics.rs:111-182), mesh_n^3 particles.
+void zeldovich_bbks(Ctx &c, const tpm_cosmology &cosmo, const PowerSpec &ps, uint64_t seed, double a_init,
+                    double *d_pos, double *d_mom);
 
 // Zel'dovich lattice ICs (ic_kernels.hip): positions and momenta (3 n^3 doubles each, lattice
 // order) on the device.

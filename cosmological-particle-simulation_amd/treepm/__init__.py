@@ -45,9 +45,16 @@ EXPORTS = [
     "tpm_device_alloc", "tpm_device_free", "tpm_memcpy", "tpm_sync", "tpm_nfw_halos", "tpm_nfw_halos_dev",
     "tpm_mlp_param_count", "tpm_mlp_init", "tpm_mlp_get_params", "tpm_mlp_set_params", "tpm_mlp_gradients",
     "tpm_mlp_predict", "tpm_mlp_train_dev", "tpm_accelerations_learned_dev",
-    "tpm_growth", "tpm_autotune", "tpm_set_tuning", "tpm_ensemble_create", "tpm_ensemble_destroy", "tpm_ensemble_last_error",
+    "tpm_growth", "tpm_power_spectrum", "tpm_power_sigma", "tpm_zeldovich_bbks", "tpm_zeldovich_bbks_dev",
+    "tpm_snapshot_dev", "tpm_run_settings_default", "tpm_run_settings_sanitized", "tpm_headless_run", "tpm_downsample_grid_dev", "tpm_autotune", "tpm_set_tuning", "tpm_ensemble_create", "tpm_ensemble_destroy", "tpm_ensemble_last_error",
     "tpm_ensemble_ics", "tpm_ensemble_autotune", "tpm_ensemble_run", "tpm_ensemble_get",
 ]
+
+
+# The reference's spectrum constants (src/ics.rs:14-18) and Hubble constant (src/cosmology.rs:27).
+N_S = 0.965
+SIGMA8 = 0.811
+H0 = 67.4
 
 
 class TpmError(RuntimeError):
@@ -132,6 +139,38 @@ class EnsembleParams(ctypes.Structure):
                 ("softening_scale", ctypes.c_double), ("theta", ctypes.c_double)]
 
 
+class RunSettings(ctypes.Structure):
+    """RunSettings (src/runner.rs:29-47)."""
+    _fields_ = [("z_start", ctypes.c_double), ("z_end", ctypes.c_double), ("steps", ctypes.c_int32),
+                ("particles_per_side", ctypes.c_int32), ("seed", ctypes.c_uint64), ("treepm", ctypes.c_int32),
+                ("theta", ctypes.c_double), ("softening_scale", ctypes.c_double), ("pacing_ms", ctypes.c_uint32)]
+
+    @classmethod
+    def default(cls):
+        s = cls()
+        load().tpm_run_settings_default(ctypes.byref(s))
+        return s
+
+    def sanitized(self):
+        out = RunSettings()
+        rc = load().tpm_run_settings_sanitized(ctypes.byref(self), ctypes.byref(out))
+        if rc != TPM_OK:
+            raise TpmError(rc, "settings are not finite")
+        return out
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Frame(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_int32), ("steps", ctypes.c_int32), ("redshift", ctypes.c_double),
+                ("max_speed", ctypes.c_double), ("elapsed_s", ctypes.c_double), ("n", ctypes.c_size_t),
+                ("d_pos", ctypes.c_void_p), ("d_speed", ctypes.c_void_p), ("d_density", ctypes.c_void_p),
+                ("d_rsd", ctypes.c_void_p), ("d_delta", ctypes.c_void_p), ("mesh_n", ctypes.c_int32)]
+
+
+FRAME_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(Frame))
+
 _lib = None
 
 
@@ -183,6 +222,16 @@ def load():
         "tpm_mlp_train_dev": (ctypes.c_int, [P, P, S, D, S, ctypes.c_int, S, F, ctypes.c_uint64, P]),
         "tpm_accelerations_learned_dev": (ctypes.c_int, [P, P, S, D, D, P, ctypes.POINTER(ctypes.c_uint64)]),
         "tpm_growth": (ctypes.c_int, [ctypes.POINTER(Cosmology), D, P, P, P]),
+        "tpm_power_spectrum": (ctypes.c_int, [ctypes.POINTER(Cosmology), D, D, D, D, P, P]),
+        "tpm_power_sigma": (ctypes.c_int, [ctypes.POINTER(Cosmology), D, D, D, D, P]),
+        "tpm_zeldovich_bbks": (ctypes.c_int, [P, ctypes.POINTER(Cosmology), D, D, D, ctypes.c_uint64, D, P, P]),
+        "tpm_zeldovich_bbks_dev": (ctypes.c_int, [P, ctypes.POINTER(Cosmology), D, D, D, ctypes.c_uint64, D, P, P,
+                                                  P]),
+        "tpm_run_settings_default": (None, [ctypes.POINTER(RunSettings)]),
+        "tpm_run_settings_sanitized": (ctypes.c_int, [ctypes.POINTER(RunSettings), ctypes.POINTER(RunSettings)]),
+        "tpm_headless_run": (ctypes.c_int, [ctypes.POINTER(RunSettings), FRAME_FN, P, P, P, S]),
+        "tpm_snapshot_dev": (ctypes.c_int, [P, P, P, S, D, D, P, P, P, P, P, P]),
+        "tpm_downsample_grid_dev": (ctypes.c_int, [P, P, P, ctypes.POINTER(ctypes.c_int32), P]),
         "tpm_autotune": (ctypes.c_int, [P, P, S, D, P, ctypes.POINTER(Tuning)]),
         "tpm_set_tuning": (ctypes.c_int, [P, ctypes.POINTER(Tuning)]),
         "tpm_ensemble_create": (ctypes.c_int, [ctypes.POINTER(EnsembleParams), ctypes.c_int, P,
@@ -360,6 +409,22 @@ class Solver:
                                            _ptr(pos), _ptr(mom)))
         return pos, mom
 
+    def zeldovich_bbks(self, seed, a_init, cosmo=None, h0=H0, n_s=N_S, sigma8=SIGMA8):
+        """zeldovich (src/ics.rs:111-182) with the sigma_8-normalised BBKS spectrum: (pos, mom), FP64."""
+        cosmo = cosmo or Cosmology.lcdm_default()
+        npart = self.n ** 3
+        pos = np.empty((npart, 3), np.float64)
+        mom = np.empty((npart, 3), np.float64)
+        self._check(self.lib.tpm_zeldovich_bbks(self.h, ctypes.byref(cosmo), float(h0), float(n_s), float(sigma8),
+                                                int(seed), float(a_init), _ptr(pos), _ptr(mom)))
+        return pos, mom
+
+    def zeldovich_bbks_dev(self, seed, a_init, d_pos, d_mom=0, stream=0, cosmo=None, h0=H0, n_s=N_S, sigma8=SIGMA8):
+        cosmo = cosmo or Cosmology.lcdm_default()
+        self._check(self.lib.tpm_zeldovich_bbks_dev(self.h, ctypes.byref(cosmo), float(h0), float(n_s),
+                                                    float(sigma8), int(seed), float(a_init), ctypes.c_void_p(d_pos),
+                                                    ctypes.c_void_p(d_mom or None), ctypes.c_void_p(stream or None)))
+
     def zeldovich_dev(self, seed, k_c, disp_rms, mom_factor, d_pos, d_mom=0, stream=0):
         self._check(self.lib.tpm_zeldovich_dev(self.h, int(seed), float(k_c), float(disp_rms), float(mom_factor),
                                                ctypes.c_void_p(d_pos), ctypes.c_void_p(d_mom or None),
@@ -432,6 +497,20 @@ class Solver:
 
     def sync(self):
         self._check(self.lib.tpm_sync(self.h))
+
+    def snapshot_dev(self, d_pos, d_mom, n, a, e, d_delta, d_pos32=0, d_speed=0, d_density=0, d_rsd=0, stream=0):
+        """snapshot (src/runner.rs:262-284) into device buffers (0 = not wanted)."""
+        v = lambda x: ctypes.c_void_p(x or None)  # noqa: E731
+        self._check(self.lib.tpm_snapshot_dev(self.h, v(d_pos), v(d_mom), int(n), float(a), float(e), v(d_delta),
+                                              v(d_pos32), v(d_speed), v(d_density), v(d_rsd), v(stream)))
+
+    def downsample_grid_dev(self, d_delta, d_out=0, stream=0):
+        """downsample_grid (src/frame.rs:33-52); returns the output side m."""
+        m = ctypes.c_int32()
+        self._check(self.lib.tpm_downsample_grid_dev(self.h, ctypes.c_void_p(d_delta or None),
+                                                     ctypes.c_void_p(d_out or None), ctypes.byref(m),
+                                                     ctypes.c_void_p(stream or None)))
+        return m.value
 
     def autotune(self, d_pos, n, a, d_acc):
         """Times the FFT and walk-group knobs on this input and keeps the fastest (returns the choice)."""
@@ -516,6 +595,33 @@ class Ensemble:
             pass
 
 
+def headless_run(settings, on_frame=None, want_state=False):
+    """The `--no-viz` run (src/runner.rs:186-260): calls on_frame(frame) after every step (frame
+    fields as in tpm_frame; device pointers valid during the call; return True to stop). Returns
+    (pos, mom) of the final state when want_state, else None."""
+    lib = load()
+    s = settings.sanitized()
+    npart = s.particles_per_side ** 3
+    errors = []
+
+    def cb(_user, fp):
+        try:
+            return 1 if (on_frame is not None and on_frame(fp.contents)) else 0
+        except Exception as e:  # surfaced after the run
+            errors.append(e)
+            return 1
+
+    fn = FRAME_FN(cb)
+    pos = np.empty((npart, 3), np.float64) if want_state else None
+    mom = np.empty((npart, 3), np.float64) if want_state else None
+    rc = lib.tpm_headless_run(ctypes.byref(settings), fn, None, _ptr(pos), _ptr(mom), npart if want_state else 0)
+    if errors:
+        raise errors[0]
+    if rc != TPM_OK:
+        raise TpmError(rc, lib.tpm_last_error(None).decode())
+    return (pos, mom) if want_state else None
+
+
 def coupling(omega_m, box_size, a, n):
     """K = 1.5 Omega_m L^3 / (4 pi a N) in float32 (src/integrator.rs:150-152)."""
     f = np.float32
@@ -541,6 +647,28 @@ def growth(a, cosmo=None):
     if rc != TPM_OK:
         raise TpmError(rc, "invalid growth arguments")
     return tuple(o.value for o in out)
+
+
+def power_spectrum(k, cosmo=None, h0=H0, n_s=N_S, sigma8=SIGMA8):
+    """(P(k), amplitude) of the sigma_8-normalised BBKS spectrum (src/ics.rs:20-64), FP64."""
+    cosmo = cosmo or Cosmology.lcdm_default()
+    p, amp = ctypes.c_double(), ctypes.c_double()
+    rc = load().tpm_power_spectrum(ctypes.byref(cosmo), float(h0), float(n_s), float(sigma8), float(k),
+                                   ctypes.byref(p), ctypes.byref(amp))
+    if rc != TPM_OK:
+        raise TpmError(rc, "invalid spectrum arguments")
+    return p.value, amp.value
+
+
+def power_sigma(radius, cosmo=None, h0=H0, n_s=N_S, sigma8=SIGMA8):
+    """sigma(R) of the normalised spectrum (PowerSpectrum::sigma, src/ics.rs:35-52)."""
+    cosmo = cosmo or Cosmology.lcdm_default()
+    out = ctypes.c_double()
+    rc = load().tpm_power_sigma(ctypes.byref(cosmo), float(h0), float(n_s), float(sigma8), float(radius),
+                                ctypes.byref(out))
+    if rc != TPM_OK:
+        raise TpmError(rc, "invalid spectrum arguments")
+    return out.value
 
 
 def zeldovich_momentum_factor(a, cosmo=None):

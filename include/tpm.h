@@ -226,6 +226,83 @@ typedef struct tpm_cosmology {
 } tpm_cosmology;
 
 /*
+ * The reference's default IC spectrum (src/ics.rs:20-64): P(k) = A k^n_s T_BBKS(k)^2 with
+ * q = k / (Omega_m h0 / 100) (src/cosmology.rs:93-103), A set so that sigma(8 Mpc/h) = sigma8 (top-hat,
+ * Simpson in ln k over [1e-4, 1e3], 8192 intervals), all in FP64. No device work.
+ * tpm_power_spectrum gives P(k) and A; tpm_power_sigma gives sigma(radius) of the normalised spectrum.
+ */
+int tpm_power_spectrum(const tpm_cosmology *cosmo, double h0, double n_s, double sigma8, double k, double *p,
+                       double *amplitude);
+int tpm_power_sigma(const tpm_cosmology *cosmo, double h0, double n_s, double sigma8, double radius, double *sigma);
+/*
+ * zeldovich (src/ics.rs:111-182) with that spectrum: mesh_n^3 lattice particles at scale factor a_init,
+ * x = q + D(a) Psi wrapped into [0, L), p = a^2 f E D Psi, from the generator's own seeded noise.
+ */
+int tpm_zeldovich_bbks(tpm_ctx *ctx, const tpm_cosmology *cosmo, double h0, double n_s, double sigma8, uint64_t seed,
+                       double a_init, double *pos, double *mom);
+int tpm_zeldovich_bbks_dev(tpm_ctx *ctx, const tpm_cosmology *cosmo, double h0, double n_s, double sigma8,
+                           uint64_t seed, double a_init, double *d_pos, double *d_mom, void *stream);
+
+/*
+ * Per-step outputs of the runner's snapshot (src/runner.rs:262-284) for FP64 device state, all
+ * device pointers, asynchronous on `stream` (NULL: the context's): positions as float (3n),
+ * speeds |p| / a (n), densities max(1 + delta(x), 0) CIC-sampled from the n_mesh^3 density-contrast
+ * mesh d_delta (n), redshift-space shifts p / (a^2 E) (frame.rs:56-64, 3n). Any output may be NULL.
+ */
+int tpm_snapshot_dev(tpm_ctx *ctx, const double *d_pos, const double *d_mom, size_t n, double a, double e,
+                     const float *d_delta, float *d_pos32, float *d_speed, float *d_density, float *d_rsd,
+                     void *stream);
+/*
+ * downsample_grid (src/frame.rs:33-52): 1 + delta averaged over f^3 blocks, f = ceil(n_mesh / 128),
+ * side m = n_mesh / f, x fastest. *side receives m; with d_out NULL nothing else is done.
+ */
+int tpm_downsample_grid_dev(tpm_ctx *ctx, const float *d_delta, float *d_out, int32_t *side, void *stream);
+
+/*
+ * Headless runner: RunSettings (src/runner.rs:29-82) and the worker loop of the `--no-viz` run
+ * (src/runner.rs:186-260, src/main.rs:34-37) on the GPU. tpm_run_settings_default gives
+ * RunSettings::default, tpm_run_settings_sanitized applies RunSettings::sanitized's clamps (FP32).
+ * tpm_headless_run sanitizes, builds the ICs (BBKS/sigma_8 Zel'dovich at z_start, seed), runs
+ * `steps` KDK steps evenly spaced in ln a on a mesh of twice the particle count per side (TreePM
+ * with ForceSplit::with when treepm != 0) and calls on_frame after each step with the snapshot in
+ * device memory (valid during the call); a non-zero return stops the run. The final positions and
+ * momenta (3 N doubles each, N = particles_per_side^3 <= capacity) are copied out when given.
+ */
+typedef struct tpm_run_settings {
+    double z_start;
+    double z_end;
+    int32_t steps;
+    int32_t particles_per_side;
+    uint64_t seed;
+    int32_t treepm;
+    double theta;
+    double softening_scale;
+    uint32_t pacing_ms; /* kept for the mirror; the headless loop does not pause */
+} tpm_run_settings;
+
+typedef struct tpm_frame {
+    int32_t step;
+    int32_t steps;
+    double redshift;
+    double max_speed; /* the run's blow-up check: max |p| / a */
+    double elapsed_s;
+    size_t n;
+    const float *d_pos;     /* 3N: positions as float */
+    const float *d_speed;   /* N: |p| / a */
+    const float *d_density; /* N: max(1 + delta, 0) at the particles */
+    const float *d_rsd;     /* 3N: p / (a^2 E) */
+    const float *d_delta;   /* mesh_n^3 density contrast of the step's force evaluation */
+    int32_t mesh_n;
+} tpm_frame;
+
+typedef int (*tpm_frame_fn)(void *user, const tpm_frame *frame);
+
+void tpm_run_settings_default(tpm_run_settings *settings);
+int tpm_run_settings_sanitized(const tpm_run_settings *in, tpm_run_settings *out);
+int tpm_headless_run(const tpm_run_settings *settings, tpm_frame_fn on_frame, void *user, double *final_pos,
+                     double *final_mom, size_t capacity);
+
+/*
  * Autotuning (no reference counterpart): times the candidates of two knobs on the given input and
  * keeps the fastest (the default is a candidate): the cap on FFT lines per workgroup and the
  * octree level of the walk groups' cubes (as an offset from the default level).

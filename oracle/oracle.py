@@ -58,6 +58,11 @@ def lib():
             sig("ora_sim_run", None, [_P, _P, _I64, _I, R, R, R, R, R, R, _I, _I, R, R, R, R])
         L.ora_threads.restype = _I
         L.ora_set_threads.argtypes = [_I]
+        L.ora_downsample_grid.restype = _I
+        L.ora_downsample_grid.argtypes = [_P, _I, _P]
+        L.ora_snapshot_f64.restype = None
+        L.ora_snapshot_f64.argtypes = [_P, _P, _I64, ctypes.c_double, ctypes.c_double, _P, _I, ctypes.c_double,
+                                       _P, _P, _P, _P]
         _lib = L
     return _lib
 
@@ -265,3 +270,27 @@ def sim_run(pos, mom, n, box_size, a_start, a_end, steps, split=None, cosmo=None
                                       steps, 1 if split else 0, sp.get("r_split", 0.0), sp.get("r_cut", 0.0),
                                       sp.get("softening", 0.0), sp.get("theta", 0.0))
     return pos, mom
+
+
+def downsample_grid(delta, n):
+    """frame.rs:33-52 downsample_grid: (m, grid of m^3 floats, x fastest)."""
+    d = _arr(delta, np.float32).reshape(-1)
+    m = lib().ora_downsample_grid(_p(d), int(n), None)
+    out = np.empty(m ** 3, np.float32)
+    lib().ora_downsample_grid(_p(d), int(n), _p(out))
+    return m, out
+
+
+def snapshot(pos, mom, a, e, delta, n, box_size):
+    """runner.rs:262-284 snapshot of FP64 state: (positions f32, speeds, densities, rsd)."""
+    pos = _arr(pos, np.float64).reshape(-1, 3)
+    mom = _arr(mom, np.float64).reshape(-1, 3)
+    d = _arr(delta, np.float32).reshape(-1)
+    np_ = len(pos)
+    opos = np.empty((np_, 3), np.float32)
+    ospeed = np.empty(np_, np.float32)
+    odens = np.empty(np_, np.float32)
+    orsd = np.empty((np_, 3), np.float32)
+    lib().ora_snapshot_f64(_p(pos), _p(mom), np_, float(a), float(e), _p(d), int(n), float(box_size), _p(opos),
+                           _p(ospeed), _p(odens), _p(orsd))
+    return opos, ospeed, odens, orsd

@@ -199,6 +199,27 @@ def test_zeldovich_matches_numpy_twin(treepm):
     assert abs(np.sqrt((disp ** 2).sum(1).mean()) - 0.3 * 100.0 / n) < 1e-9
 
 
+def test_bbks_zeldovich_matches_numpy_twin(treepm):
+    # The reference's default ICs (src/runner.rs:194-206): sigma_8-normalised BBKS spectrum at z = 500,
+    # x = q + D Psi, p = a^2 f E D Psi; the same noise through numpy's FFT.
+    import ic_twin
+    n, box = 16, 100.0
+    a0 = 1.0 / 501.0
+    s = solver(treepm, n, box, max_particles=n ** 3)
+    pos, mom = s.zeldovich_bbks(5, a0)
+    _, amp = treepm.power_spectrum(0.1)
+    gamma = treepm.Cosmology.lcdm_default().omega_m * treepm.H0 / 100.0
+    d, f, e = treepm.growth(a0)
+    ref, rmom = ic_twin.zeldovich_bbks(n, box, 5, amp, treepm.N_S, gamma, d, a0 * a0 * f * e)
+    dd = pos - ref
+    dd -= box * np.round(dd / box)
+    assert np.max(np.abs(dd)) < 1e-9 * box
+    assert np.max(np.abs(mom - rmom)) < 1e-9 * np.abs(rmom).max()
+    # the displacement at z = 500 is a small fraction of a cell (D ~ 0.0026)
+    disp = np.sqrt((mom / (a0 * a0 * f * e) ** 1) ** 2).max()
+    assert 0 < disp < 0.5 * box / n
+
+
 @pytest.mark.parametrize("n", [16, 32])
 def test_full_treepm_matches_oracle(treepm, oracle, n):
     pos = _zeldovich_config(treepm, n).astype(np.float32)

@@ -89,3 +89,9 @@ def test_library_growth_matches_oracle(oracle, treepm, a):
     assert abs(f - oracle.growth_rate(a, dtype=np.float64)) < 1e-12
     assert abs(e - oracle.cosmo_e(a, dtype=np.float64)) < 1e-12 * e
     assert treepm.zeldovich_momentum_factor(a) == pytest.approx(a * a * f * e, rel=1e-15)
+
+
+def test_growth_rate_matches_linder_today(treepm):
+    # src/ics.rs:202-208: f(1) ~ Omega_m^0.55 to 0.01.
+    _, f, _ = treepm.growth(1.0)
+    assert abs(f - treepm.Cosmology.lcdm_default().omega_m ** 0.55) < 0.01
