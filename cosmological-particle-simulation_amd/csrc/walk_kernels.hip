@@ -47,13 +47,13 @@ struct WalkArgs {
     float boxf;
     float invboxf;
     float cut2;
+    float cut2s;  // r_cut^2 + eps^2 (the pair cutoff on s2 = r^2 + eps^2)
     float eps2;
     float inv2rs;
     float negk;
-    // The S table is indexed by x = sqrt(r^2 + eps^2) * c (c = kSTab / rho_max): s' = r^2 c^2 + eps^2 c^2
+    // The S table is indexed by x = sqrt(r^2 + eps^2) * c (c = kSTab / rho_max): s' = (r^2 + eps^2) c^2
     // gives x = s' rsqrt(s') and 1 / (r^2 + eps^2)^{3/2} = c^3 rsqrt(s')^3, so the table carries -K c^3.
     float sc2;   // c^2
-    float e2s;   // eps^2 c^2
     float tabk;  // -K c^3
     const float2 *stab;
     uint32_t *next_group;
@@ -170,9 +170,9 @@ __device__ __forceinline__ void wave_sync() {
 // interpolation error of S below 1e-5). One reciprocal square root gives both the table index
 // and the softened 1 / (r^2 + eps^2)^{3/2}. Used for particle pairs, which satisfy r <= r_cut, so
 // the interval index is at most kSTab - 1 and needs no clamp.
-__device__ __forceinline__ void pair_tab(float dx, float dy, float dz, float r2, const float2 *stab,
+__device__ __forceinline__ void pair_tab(float dx, float dy, float dz, float s2, const float2 *stab,
                                          const WalkArgs &w, float &ax, float &ay, float &az) {
-    const float s = fmaf(r2, w.sc2, w.e2s);
+    const float s = s2 * w.sc2;
     const float rs = __builtin_amdgcn_rsqf(s);
     const float x = s * rs;
     const float fr = __builtin_amdgcn_fractf(x);
@@ -196,12 +196,13 @@ __device__ __forceinline__ void stage_one(const float4 src, const float2 *stab, 
         dy -= w.boxf * rintf(dy * w.invboxf);
         dz -= w.boxf * rintf(dz * w.invboxf);
     }
-    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    bool take = active && r2 <= w.cut2;
+    // s2 = r^2 + eps^2; the cutoff r^2 <= r_cut^2 is tested as s2 <= r_cut^2 + eps^2.
+    const float s2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, w.eps2)));
+    bool take = active && s2 <= w.cut2s;
     if (SELF) take = take && __float_as_uint(src.w) != self;
     if (take) {
-        pair_tab(dx, dy, dz, r2, stab, w, ax, ay, az);
-        ++ninter;
+        pair_tab(dx, dy, dz, s2, stab, w, ax, ay, az);
+        ++ninter;  // fills the wait state after the reciprocal square root: costs nothing
     }
 }
 
@@ -687,11 +688,11 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     w.invboxf = (float)(1.0 / c.p.box_size);
     w.cut2 = (float)c.p.r_cut * (float)c.p.r_cut;
     w.eps2 = (float)c.p.softening * (float)c.p.softening;
+    w.cut2s = w.cut2 + w.eps2;
     w.inv2rs = (float)(1.0 / (2.0 * c.p.r_split));
     w.negk = (float)(-k);
     const double sc = c.stab_scale;
     w.sc2 = (float)(sc * sc);
-    w.e2s = (float)(c.p.softening * c.p.softening * sc * sc);
     w.tabk = (float)(-k * sc * sc * sc);
     w.stab = c.stab;
     w.acc = d_acc;
