@@ -67,22 +67,31 @@ static double tune_eval_ms(Ctx &c, const double *d_pos, size_t n, double a, floa
 
 tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_acc) {
     tpm_tuning t{};
-    const int cap0 = c.fft_cap, shift0 = c.group_level_shift;
+    const int cap0 = c.fft_cap, shift0 = c.group_level_shift, path0 = c.fft_path;
     double f0, w0;
     t.ms_default = tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f0, &w0);
-    int best_cap = cap0;
+    int best_cap = cap0, best_path = path0;
     double best_f = f0;
-    for (int cap : {2, 4, 8, 16}) {
-        if (cap == cap0) continue;
-        c.fft_cap = cap;
-        double f, w;
-        tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f, &w);
-        if (f < best_f) {
-            best_f = f;
-            best_cap = cap;
+    // FFT kernels: the wavefront transforms (mesh side 512 only) or the LDS line kernels with each
+    // cap on lines per workgroup.
+    for (int path : {0, 1}) {
+        if (path == 0 && c.n != 512) continue;
+        for (int cap : {2, 4, 8, 16}) {
+            if (path == 0 && cap != cap0) continue;  // the cap does not apply to the wavefront path
+            if (path == path0 && cap == cap0) continue;
+            c.fft_path = path;
+            c.fft_cap = cap;
+            double f, w;
+            tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f, &w);
+            if (f < best_f) {
+                best_f = f;
+                best_cap = cap;
+                best_path = path;
+            }
         }
     }
     c.fft_cap = best_cap;
+    c.fft_path = best_path;
     int best_shift = shift0;
     double best_w = w0;
     for (int sh : {-1, 1}) {
@@ -96,13 +105,16 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
     }
     c.group_level_shift = best_shift;
     t.fft_cap = best_cap;
+    t.fft_path = best_path;
     t.group_level_shift = best_shift;
     t.ms_tuned = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr);
     if (t.ms_tuned > t.ms_default) {
         // Timing noise can make the tuned choice lose on the final check: keep the default then.
         c.fft_cap = cap0;
+        c.fft_path = path0;
         c.group_level_shift = shift0;
         t.fft_cap = cap0;
+        t.fft_path = path0;
         t.group_level_shift = shift0;
         t.ms_tuned = t.ms_default;
     }
@@ -223,6 +235,7 @@ int tpm_ensemble_autotune(tpm_ensemble *e, double a, tpm_tuning *out) {
             } else {
                 bx.tuning = E.boxes[first].tuning;
                 c.fft_cap = bx.tuning.fft_cap;
+                c.fft_path = bx.tuning.fft_path;
                 c.group_level_shift = bx.tuning.group_level_shift;
             }
             if (out) out[b] = bx.tuning;
@@ -363,8 +376,9 @@ int tpm_autotune(tpm_ctx *ctx, const double *d_pos, size_t n, double a, float *d
 int tpm_set_tuning(tpm_ctx *ctx, const tpm_tuning *t) {
     if (!ctx || !t) return TPM_E_INVALID;
     Ctx &c = tpm_ctx_internal(ctx);
-    if (t->fft_cap < 1 || t->fft_cap > 64) return TPM_E_INVALID;
+    if (t->fft_cap < 1 || t->fft_cap > 64 || t->fft_path < 0 || t->fft_path > 1) return TPM_E_INVALID;
     c.fft_cap = t->fft_cap;
+    c.fft_path = t->fft_path;
     c.group_level_shift = t->group_level_shift;
     return TPM_OK;
 }

@@ -422,6 +422,265 @@ __global__ void __launch_bounds__(256) k_fft_z_inv3(const typename Cx<R>::T *A, 
     }
 }
 
+// ------------------------------------------------------------ 512-point wavefront transforms
+//
+// For n = 512 every line transform runs in one wavefront with the four-step split 512 = 8 x 64:
+// lane t holds x[t + 64 q] (q = 0..7, one coalesced load per q), computes the 8-point DFT over q
+// and multiplies by w512^(t k1); the 64-point DFTs over t run as 8 x 8 through the wavefront's own
+// LDS line (two exchanges); lane l ends with X[l + 64 j], j = 0..7, which is again the layout a
+// following transform starts from. Forward transforms use exp(-2 pi i nk / n), inverse ones
+// exp(+...), both unnormalised (src/fft.rs:27-35). Mesh columns (the y and x passes) are staged
+// through LDS in tiles of 8 columns so that global accesses run along kz.
+namespace w512 {
+
+constexpr int kN = 512;
+constexpr int kP1 = 72;   // pitch of the [k1][t] exchange (conflict-free for 16-lane groups)
+constexpr int kP2 = 9;    // pitch of the [m][k1][t_a] exchange
+constexpr int kLP = 585;  // LDS floats2 per line (>= 8 kP1 and 64 kP2; 585 mod 16 spreads columns)
+constexpr int kCols = 8;  // mesh columns per workgroup in the column passes
+
+__device__ __forceinline__ float2 f2(float x, float y) { return make_float2(x, y); }
+__device__ __forceinline__ float2 add(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 sub(float2 a, float2 b) { return f2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 mul(float2 a, float2 b) { return f2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// a * conj(b)
+__device__ __forceinline__ float2 mulc(float2 a, float2 b) { return f2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
+
+template <bool INV>
+__device__ __forceinline__ float2 twm(float2 z, float2 w) {
+    return INV ? mulc(z, w) : mul(z, w);
+}
+
+// U[k] = sum_n u[n] w4^(nk), w4 = s i (s = -1 forward, +1 inverse).
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 &u0, float2 &u1, float2 &u2, float2 &u3) {
+    const float2 a = add(u0, u2), b = sub(u0, u2), c = add(u1, u3), d = sub(u1, u3);
+    const float2 wd = INV ? f2(-d.y, d.x) : f2(d.y, -d.x);
+    u0 = add(a, c);
+    u2 = sub(a, c);
+    u1 = add(b, wd);
+    u3 = sub(b, wd);
+}
+
+// V[k] = sum_n v[n] w8^(nk), w8 = exp(s 2 pi i / 8), in place (radix-2 split into even and odd).
+template <bool INV>
+__device__ __forceinline__ void dft8(float2 v[8]) {
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    const float c = 0.70710678118654752f;
+    // o_k w8^k: w8 = (c, s c), w8^2 = s i, w8^3 = (-c, s c)
+    const float2 t1 = INV ? f2(c * (o1.x - o1.y), c * (o1.y + o1.x)) : f2(c * (o1.x + o1.y), c * (o1.y - o1.x));
+    const float2 t2 = INV ? f2(-o2.y, o2.x) : f2(o2.y, -o2.x);
+    const float2 t3 = INV ? f2(-c * (o3.x + o3.y), c * (o3.x - o3.y)) : f2(c * (o3.y - o3.x), -c * (o3.x + o3.y));
+    v[0] = add(e0, o0);
+    v[4] = sub(e0, o0);
+    v[1] = add(e1, t1);
+    v[5] = sub(e1, t1);
+    v[2] = add(e2, t2);
+    v[6] = sub(e2, t2);
+    v[3] = add(e3, t3);
+    v[7] = sub(e3, t3);
+}
+
+// Per-lane twiddles of the forward transform: a[k] = w512^(t k1), b[k] = w64^(t_a m) (index 0: 1).
+struct Tw {
+    float2 a[8];
+    float2 b[8];
+};
+
+__device__ __forceinline__ void load_tw(Tw &T, const float2 *__restrict__ tw, int lane) {
+    T.a[0] = T.b[0] = f2(1.f, 0.f);
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+        T.a[k] = tw[(lane * k) & (kN - 1)];
+        T.b[k] = tw[(8 * (lane & 7) * k) & (kN - 1)];
+    }
+}
+
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// v[q] = x[lane + 64 q] in, v[j] = X[lane + 64 j] out; `line` is this wavefront's LDS scratch.
+template <bool INV>
+__device__ __forceinline__ void fft(float2 v[8], float2 *line, const Tw &T) {
+    const int lane = threadIdx.x & 63;
+    dft8<INV>(v);
+#pragma unroll
+    for (int k = 1; k < 8; ++k) v[k] = twm<INV>(v[k], T.a[k]);
+    wsync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) line[k * kP1 + lane] = v[k];
+    wsync();
+    {
+        const int ta = lane & 7, k1 = lane >> 3;
+#pragma unroll
+        for (int tb = 0; tb < 8; ++tb) v[tb] = line[k1 * kP1 + ta + 8 * tb];
+        dft8<INV>(v);
+#pragma unroll
+        for (int m = 1; m < 8; ++m) v[m] = twm<INV>(v[m], T.b[m]);
+        wsync();
+#pragma unroll
+        for (int m = 0; m < 8; ++m) line[(8 * m + k1) * kP2 + ta] = v[m];
+    }
+    wsync();
+    {
+        const int k1 = lane & 7, m = lane >> 3;
+#pragma unroll
+        for (int ta = 0; ta < 8; ++ta) v[ta] = line[(8 * m + k1) * kP2 + ta];
+        dft8<INV>(v);
+    }
+    wsync();
+}
+
+// z pass, r2c: four lines per workgroup, one per wavefront.
+__global__ void __launch_bounds__(256) k_z_fwd(ZSrc<float> src, float2 *__restrict__ spec,
+                                               const float2 *__restrict__ tw, size_t nlines) {
+    __shared__ float2 lds[4 * kLP];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t l = (size_t)xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
+    if (l >= nlines) return;  // whole wavefronts; no workgroup barrier follows
+    Tw T;
+    load_tw(T, tw, lane);
+    float2 v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const size_t gi = l * kN + lane + 64 * q;
+        const float x = zsrc_value<float>(src, gi);
+        if (src.delta_out) src.delta_out[gi] = x;
+        v[q] = f2(x, 0.f);
+    }
+    fft<false>(v, lds + wave * kLP, T);
+    float2 *out = spec + l * (kN / 2 + 1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[lane + 64 * j] = v[j];
+    if (lane == 0) out[kN / 2] = v[4];
+}
+
+// Column passes over the half spectrum (hp = 257 values of kz): AXIS 0 = y lines
+// (S[(x n + t) hp + kz] for fixed x), AXIS 1 = x lines (S[(t n + y) hp + kz] for fixed y).
+// MODE 0: forward in place; 1: inverse in place; 2: S0 <- iFFT(S0), S1 <- iFFT(S0 (-i k_t));
+// 3: forward, Green's function, then S1 <- iFFT(phi), S0 <- iFFT(phi (-i k_t)).
+template <int AXIS>
+__device__ __forceinline__ size_t col_index(int outer, int t, int kz, int hp) {
+    return AXIS == 0 ? ((size_t)outer * kN + t) * hp + kz : ((size_t)t * kN + outer) * hp + kz;
+}
+
+template <int AXIS>
+__device__ __forceinline__ void tile_store(float2 *__restrict__ S, const float2 *lds, int outer, int kz0, int hp) {
+    for (int i = threadIdx.x; i < kN * kCols; i += blockDim.x) {
+        const int c = i & (kCols - 1), t = i / kCols, kz = kz0 + c;
+        if (kz < hp) S[col_index<AXIS>(outer, t, kz, hp)] = lds[c * kLP + t];
+    }
+}
+
+template <int AXIS, int MODE>
+__global__ void __launch_bounds__(64 * kCols) k_col(float2 *__restrict__ S0, float2 *__restrict__ S1,
+                                                     const float2 *__restrict__ tw, const double *__restrict__ kvec,
+                                                     int hp, int ntiles, GreenArgs green) {
+    __shared__ float2 lds[kCols * kLP];
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const int tile = (int)(b % (uint32_t)ntiles), outer = (int)(b / (uint32_t)ntiles);
+    const int kz0 = tile * kCols;
+    for (int i = threadIdx.x; i < kN * kCols; i += blockDim.x) {
+        const int c = i & (kCols - 1), t = i / kCols, kz = kz0 + c;
+        lds[c * kLP + t] = kz < hp ? S0[col_index<AXIS>(outer, t, kz, hp)] : f2(0.f, 0.f);
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float2 *line = lds + w * kLP;
+    Tw T;
+    load_tw(T, tw, lane);
+    float2 v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = line[lane + 64 * q];
+    if (MODE == 0 || MODE == 1) {
+        fft<MODE == 1>(v, line, T);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) line[lane + 64 * j] = v[j];
+        __syncthreads();
+        tile_store<AXIS>(S0, lds, outer, kz0, hp);
+        return;
+    }
+    float2 m[8];
+    if (MODE == 2) {
+        // (v) (0 - i k_t): re = v.y k, im = -v.x k, for t = lane + 64 q
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float k = (float)kvec[lane + 64 * q];
+            m[q] = f2(v[q].y * k, -v[q].x * k);
+        }
+    } else {
+        fft<false>(v, line, T);
+        const int kz = kz0 + w;
+        const double ko = kvec[outer];
+        const double kzv = kz < hp ? kvec[kz] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const double kt = kvec[lane + 64 * j];
+            const double k2 = kt * kt + ko * ko + kzv * kzv;
+            float2 phi = f2(0.f, 0.f);
+            if (k2 != 0.0 && kz < hp) {
+                const float g = (float)green_factor(green, k2);
+                phi = f2(v[j].x * g, v[j].y * g);
+            }
+            const float ktf = (float)kt;
+            v[j] = phi;
+            m[j] = f2(phi.y * ktf, -phi.x * ktf);
+        }
+    }
+    fft<true>(v, line, T);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) line[lane + 64 * j] = v[j];
+    __syncthreads();
+    tile_store<AXIS>(MODE == 2 ? S0 : S1, lds, outer, kz0, hp);
+    __syncthreads();
+    fft<true>(m, line, T);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) line[lane + 64 * j] = m[j];
+    __syncthreads();
+    tile_store<AXIS>(MODE == 2 ? S1 : S0, lds, outer, kz0, hp);
+}
+
+// z pass, c2r of three components: A -> g[0], Cc -> g[1], Bz (-i k_z) -> g[2]; one (line,
+// component) per wavefront, the upper half of each line by Hermitian completion.
+__global__ void __launch_bounds__(256) k_z_inv3(const float2 *__restrict__ A, const float2 *__restrict__ Cc,
+                                                const float2 *__restrict__ Bz, float *__restrict__ g, size_t m,
+                                                const float2 *__restrict__ tw, const double *__restrict__ kvec,
+                                                size_t nlines) {
+    __shared__ float2 lds[4 * kLP];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t item = (size_t)xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
+    const size_t l = item / 3;
+    const int comp = (int)(item - 3 * l);
+    if (l >= nlines) return;
+    Tw T;
+    load_tw(T, tw, lane);
+    const float2 *S = (comp == 0 ? A : comp == 1 ? Cc : Bz) + l * (kN / 2 + 1);
+    float2 v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int i = lane + 64 * q;
+        const int kz = i <= kN / 2 ? i : kN - i;
+        float2 u = S[kz];
+        if (comp == 2) {
+            const float k = (float)kvec[kz];
+            u = f2(u.y * k, -u.x * k);
+        }
+        v[q] = i <= kN / 2 ? u : f2(u.x, -u.y);
+    }
+    fft<true>(v, lds + wave * kLP, T);
+    float *out = g + comp * m + l * kN;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) out[lane + 64 * j] = v[j].x;
+}
+
+}  // namespace w512
+
 // ----------------------------------------------------------------------------- host side
 
 static int factor_n(int n, int *radix) {
@@ -494,6 +753,28 @@ static void pm_pipeline(Ctx &c, const ZSrc<R> &src, typename Cx<R>::T *SA, typen
     const size_t es = sizeof(C);
     const int ld = n + 1;
     TPM_REQUIRE(2 * (size_t)ld * es <= 65536, "mesh too large for the LDS line transform");
+
+    if constexpr (sizeof(R) == 4) {
+        if (n == w512::kN && c.fft_path == 0) {
+            // Wavefront transforms: z r2c, y forward, x forward + Green + 2 inverse, y inverse
+            // (A; B -> B, C), z c2r of the three components.
+            const size_t lines = (size_t)n * n;
+            const int ntiles = (hp + w512::kCols - 1) / w512::kCols;
+            const dim3 gcol((unsigned)(ntiles * n)), bcol(64 * w512::kCols);
+            const double *kv = c.kvec;
+            hipLaunchKernelGGL(w512::k_z_fwd, dim3((unsigned)(lines / 4)), dim3(256), 0, c.stream, src, SA, tw, lines);
+            hipLaunchKernelGGL((w512::k_col<0, 0>), gcol, bcol, 0, c.stream, SA, (float2 *)nullptr, tw, kv, hp, ntiles,
+                               green);
+            hipLaunchKernelGGL((w512::k_col<1, 3>), gcol, bcol, 0, c.stream, SA, SB, tw, kv, hp, ntiles, green);
+            hipLaunchKernelGGL((w512::k_col<0, 1>), gcol, bcol, 0, c.stream, SA, (float2 *)nullptr, tw, kv, hp, ntiles,
+                               green);
+            hipLaunchKernelGGL((w512::k_col<0, 2>), gcol, bcol, 0, c.stream, SB, SC, tw, kv, hp, ntiles, green);
+            hipLaunchKernelGGL(w512::k_z_inv3, dim3((unsigned)(3 * lines / 4)), dim3(256), 0, c.stream, SA, SC, SB,
+                               (float *)gout, c.m, tw, kv, lines);
+            TPM_HIP(hipGetLastError());
+            return;
+        }
+    }
 
     // Z forward
     {

@@ -24,14 +24,6 @@ struct Stencil {
     T w[8];
 };
 
-// Workgroups are dealt to the 8 XCDs round-robin; this renumbering gives each XCD a contiguous
-// range of logical blocks, so blocks of neighbouring particles (Morton order) share its L2. It is
-// a bijection of [0, g).
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t g) {
-    const uint32_t q = g / 8, r = g % 8, x = b % 8;
-    return x * q + min(x, r) + b / 8;
-}
-
 // Periodic wrap of an unwrapped cell index (rem_euclid). For positions in [0, L) the stencil
 // corners lie in [-1, n], so one compare-and-add covers them; anything else takes the modulo.
 __device__ __forceinline__ int64_t wrap_cell(int64_t v, int n) {

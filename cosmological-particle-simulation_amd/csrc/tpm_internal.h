@@ -13,6 +13,14 @@
 
 namespace tpm {
 
+// Workgroups are dealt to the 8 XCDs round-robin; this renumbering gives each XCD a contiguous
+// range of logical blocks, so neighbouring blocks (neighbouring particles in Morton order,
+// neighbouring mesh columns) share its L2. It is a bijection of [0, g).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t g) {
+    const uint32_t q = g / 8, r = g % 8, x = b % 8;
+    return x * q + min(x, r) + b / 8;
+}
+
 // Fixed-point scale of the deterministic mass grid: one unit of mass is 2^32 counts.
 constexpr int kFixedShift = 32;
 // Morton key resolution per axis (3*21 = 63-bit keys); also the deepest octree level.
@@ -145,7 +153,8 @@ struct Ctx {
     size_t gscan_bytes = 0;
     int group_level = 2;
     int group_level_shift = 0;  // autotuned offset of the group cube level
-    int fft_cap = 16;           // autotuned cap on FFT lines per workgroup
+    int fft_cap = 16;           // autotuned cap on FFT lines per workgroup (LDS line kernels)
+    int fft_path = 0;           // 0: wavefront transforms where the mesh side has them (512), 1: LDS line kernels
     uint32_t *tree_scan = nullptr;  // per-level child counts, offsets, scan partials, level table
     uint32_t *tree_info = nullptr;  // [0] node count, [1] capacity exceeded
     uint32_t *ngroups_dev = nullptr;  // walk group count (device)

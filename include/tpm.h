@@ -303,13 +303,15 @@ int tpm_headless_run(const tpm_run_settings *settings, tpm_frame_fn on_frame, vo
                      double *final_mom, size_t capacity);
 
 /*
- * Autotuning (no reference counterpart): times the candidates of two knobs on the given input and
- * keeps the fastest (the default is a candidate): the cap on FFT lines per workgroup and the
- * octree level of the walk groups' cubes (as an offset from the default level).
+ * Autotuning (no reference counterpart): times the candidates of three knobs on the given input and
+ * keeps the fastest (the default is a candidate): the FFT kernels (wavefront transforms or LDS line
+ * kernels), the cap on FFT lines per workgroup of the latter and the octree level of the walk
+ * groups' cubes (as an offset from the default level).
  */
 typedef struct tpm_tuning {
     int32_t fft_cap;
     int32_t group_level_shift;
+    int32_t fft_path; /* 0: wavefront transforms where available (mesh side 512), 1: LDS line kernels */
     double ms_default; /* force evaluation with the default knobs */
     double ms_tuned;   /* with the chosen knobs */
 } tpm_tuning;

@@ -120,6 +120,36 @@ def test_uniform_delta_and_inverse_a(treepm):
     assert abs(r - 2.0) < 1e-4
 
 
+def test_wavefront_transforms_at_512(treepm):
+    # The n = 512 wavefront path: plane waves along each axis and an oblique one against the
+    # closed form g = -(1.5 Omega_m / a) k sin(k.x) / k^2 (src/poisson.rs:114-139), then a random
+    # field against the LDS line kernels on the same context.
+    n, a = 512, 0.5
+    c = 1.5 * OMEGA_M / a
+    print("[gpu] PM at 512^3, wavefront transforms", flush=True)
+    s = solver(treepm, n, 1.0)
+    x = (np.arange(n) + 0.5) / n
+    for kv in ((7, 0, 0), (0, 5, 0), (0, 0, 9), (3, -5, 7)):
+        k = 2 * np.pi * np.array(kv, np.float64)
+        phase = (k[0] * x[:, None, None] + k[1] * x[None, :, None] + k[2] * x[None, None, :])
+        d = np.cos(phase).astype(np.float32).ravel()
+        g = s.pm_accel(d, a, 0.0)
+        sn = np.sin(phase).ravel()
+        k2 = (k ** 2).sum()
+        amp = c / np.sqrt(k2)
+        for ax in range(3):
+            exp = -c * k[ax] * sn / k2
+            assert np.max(np.abs(g[ax] - exp)) < 2e-5 * amp, (kv, ax)
+        del g, sn, d, phase
+    rng = np.random.default_rng(512)
+    d = rng.standard_normal(n ** 3).astype(np.float32)
+    rs = 1.25 / n
+    g_wave = s.pm_accel(d, 0.3, rs)
+    s.set_tuning(16, 0, fft_path=1)
+    g_line = s.pm_accel(d, 0.3, rs)
+    assert np.max(np.abs(g_wave - g_line)) < 2e-5 * np.abs(g_line).max()
+
+
 @pytest.mark.parametrize("n", [4, 6, 8, 14, 16, 24, 30, 32, 47, 64])
 def test_pm_random_delta_matches_oracle(treepm, oracle, n):
     rng = np.random.default_rng(n)
