@@ -163,7 +163,7 @@ def ensemble_bench(args):
     log(f"[bench] ensemble: graph run {ms_graph:.1f} ms (first launch incl. capture {capture_s:.1f} s)")
     choices = {}
     for side, t in zip(sides, tuning):
-        choices[str(side)] = {"fft_path": t["fft_path"], "fft_cap": t["fft_cap"],
+        choices[str(side)] = {"fft_path": t["fft_path"], "fft_cap": t["fft_cap"], "tree_depth": t["tree_depth"],
                               "group_level_shift": t["group_level_shift"],
                               "ms_default": round(t["ms_default"], 3), "ms_tuned": round(t["ms_tuned"], 3)}
     e.close()

@@ -67,7 +67,7 @@ static double tune_eval_ms(Ctx &c, const double *d_pos, size_t n, double a, floa
 
 tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_acc) {
     tpm_tuning t{};
-    const int cap0 = c.fft_cap, shift0 = c.group_level_shift, path0 = c.fft_path;
+    const int cap0 = c.fft_cap, shift0 = c.group_level_shift, path0 = c.fft_path, depth0 = c.tree_depth;
     double f0, w0;
     t.ms_default = tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f0, &w0);
     int best_cap = cap0, best_path = path0;
@@ -104,6 +104,21 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
         }
     }
     c.group_level_shift = best_shift;
+    // Octree depth: a shallower tree sorts fewer key bits and builds fewer levels; deeper nodes
+    // stay leaves (more direct sums). Judged on the whole evaluation.
+    int best_depth = depth0;
+    double best_total = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr);
+    for (int depth : {16, 12}) {
+        if (depth == depth0) continue;
+        c.tree_depth = depth;
+        const double tt = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr);
+        if (tt < best_total) {
+            best_total = tt;
+            best_depth = depth;
+        }
+    }
+    c.tree_depth = best_depth;
+    t.tree_depth = best_depth;
     t.fft_cap = best_cap;
     t.fft_path = best_path;
     t.group_level_shift = best_shift;
@@ -113,8 +128,10 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
         c.fft_cap = cap0;
         c.fft_path = path0;
         c.group_level_shift = shift0;
+        c.tree_depth = depth0;
         t.fft_cap = cap0;
         t.fft_path = path0;
+        t.tree_depth = depth0;
         t.group_level_shift = shift0;
         t.ms_tuned = t.ms_default;
     }
@@ -236,6 +253,7 @@ int tpm_ensemble_autotune(tpm_ensemble *e, double a, tpm_tuning *out) {
                 bx.tuning = E.boxes[first].tuning;
                 c.fft_cap = bx.tuning.fft_cap;
                 c.fft_path = bx.tuning.fft_path;
+                c.tree_depth = bx.tuning.tree_depth;
                 c.group_level_shift = bx.tuning.group_level_shift;
             }
             if (out) out[b] = bx.tuning;
@@ -377,6 +395,8 @@ int tpm_set_tuning(tpm_ctx *ctx, const tpm_tuning *t) {
     if (!ctx || !t) return TPM_E_INVALID;
     Ctx &c = tpm_ctx_internal(ctx);
     if (t->fft_cap < 1 || t->fft_cap > 64 || t->fft_path < 0 || t->fft_path > 1) return TPM_E_INVALID;
+    if (t->tree_depth < 10 || t->tree_depth > tpm::kMaxDepth) return TPM_E_INVALID;
+    c.tree_depth = t->tree_depth;
     c.fft_cap = t->fft_cap;
     c.fft_path = t->fft_path;
     c.group_level_shift = t->group_level_shift;

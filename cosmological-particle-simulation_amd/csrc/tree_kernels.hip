@@ -84,14 +84,14 @@ constexpr int kScanBlocks = 256;
 
 // Pass 1 over one level: decide which nodes split and count their non-empty octants.
 __global__ void __launch_bounds__(256) k_count_children(const uint64_t *__restrict__ keys, TreeView t,
-                                                        const uint32_t *__restrict__ lv, int level,
+                                                        const uint32_t *__restrict__ lv, int level, int max_depth,
                                                         uint32_t ref_leaf_rule, uint32_t *__restrict__ nchild) {
     const uint32_t lb = lv[level], le = lv[level + 1];
     for (uint32_t q = lb + blockIdx.x * blockDim.x + threadIdx.x; q < le; q += gridDim.x * blockDim.x) {
         const uint2 r = t.range[q];
         const uint32_t cnt = r.y - r.x;
         uint32_t mask = 0, nc = 0;
-        if (cnt > (uint32_t)kLeafSize && level < kMaxDepth) {
+        if (cnt > (uint32_t)kLeafSize && level < max_depth) {
             uint32_t b[9];
             b[0] = r.x;
             b[8] = r.y;
@@ -282,7 +282,10 @@ void sort_particles(Ctx &c, const double *d_pos, size_t n) {
     rocprim::double_buffer<uint64_t> kb(c.keys, c.keys_alt);
     rocprim::double_buffer<uint32_t> vb(c.idx, c.idx_alt);
     size_t bytes = c.sort_tmp_bytes;
-    TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 0, 3 * kKeyBits, c.stream));
+    // Only the key bits of the octree's levels are sorted (a shallower tree sorts fewer bits; the
+    // sort is stable, so the order below that depth is the input order).
+    TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 3 * (kKeyBits - c.tree_depth), 3 * kKeyBits,
+                                      c.stream));
     if (kb.current() != c.keys) std::swap(c.keys, c.keys_alt);
     if (vb.current() != c.idx) std::swap(c.idx, c.idx_alt);
     hipLaunchKernelGGL(k_gather, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.spos);
@@ -325,9 +328,9 @@ void build_tree(Ctx &c, size_t n) {
     const uint32_t ref_rule = (c.p.flags & TPM_FLAG_REF_LEAF_RULE) ? 1u : 0u;
     // Node-parallel kernels stride over each level with a grid sized for n particles.
     const unsigned grid = std::max(1u, std::min(blocks(n, 256), 4096u));
-    for (int level = 0; level <= kMaxDepth; ++level) {
-        hipLaunchKernelGGL(k_count_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level, ref_rule,
-                           nch);
+    for (int level = 0; level <= c.tree_depth; ++level) {
+        hipLaunchKernelGGL(k_count_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level,
+                           c.tree_depth, ref_rule, nch);
         hipLaunchKernelGGL(k_scan_partials, dim3(kScanBlocks), dim3(256), 0, c.stream, nch, lv, level, part);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlocks), 0, c.stream, part, lv, level, (uint32_t)t.cap,
                            info);
@@ -337,7 +340,7 @@ void build_tree(Ctx &c, size_t n) {
         TPM_HIP(hipGetLastError());
     }
     // Centres of mass, deepest level first (levels past the deepest are empty ranges).
-    for (int level = kMaxDepth + 1; level >= 0; --level) {
+    for (int level = c.tree_depth + 1; level >= 0; --level) {
         hipLaunchKernelGGL(k_com_level, dim3(grid), dim3(256), 0, c.stream, c.spos, t.view(), lv, level);
         TPM_HIP(hipGetLastError());
     }

@@ -305,13 +305,14 @@ int tpm_headless_run(const tpm_run_settings *settings, tpm_frame_fn on_frame, vo
 /*
  * Autotuning (no reference counterpart): times the candidates of three knobs on the given input and
  * keeps the fastest (the default is a candidate): the FFT kernels (wavefront transforms or LDS line
- * kernels), the cap on FFT lines per workgroup of the latter and the octree level of the walk
- * groups' cubes (as an offset from the default level).
+ * kernels), the cap on FFT lines per workgroup of the latter, the octree level of the walk
+ * groups' cubes (as an offset from the default level) and the octree depth limit.
  */
 typedef struct tpm_tuning {
     int32_t fft_cap;
     int32_t group_level_shift;
     int32_t fft_path; /* 0: wavefront transforms where available (mesh side 512), 1: LDS line kernels */
+    int32_t tree_depth; /* octree depth limit, 10..21 (default 21): deeper nodes stay leaves and fewer key bits are sorted */
     double ms_default; /* force evaluation with the default knobs */
     double ms_tuned;   /* with the chosen knobs */
 } tpm_tuning;
