@@ -437,7 +437,7 @@ constexpr int kN = 512;
 constexpr int kP1 = 72;   // pitch of the [k1][t] exchange (conflict-free for 16-lane groups)
 constexpr int kP2 = 9;    // pitch of the [m][k1][t_a] exchange
 constexpr int kLP = 585;  // LDS floats2 per line (>= 8 kP1 and 64 kP2; 585 mod 16 spreads columns)
-constexpr int kCols = 8;  // mesh columns per workgroup in the column passes
+constexpr int kCols = 8;  // mesh columns per workgroup in the column passes (one wavefront each)
 
 __device__ __forceinline__ float2 f2(float x, float y) { return make_float2(x, y); }
 __device__ __forceinline__ float2 add(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
