@@ -6,7 +6,8 @@
 // wave walks the octree once for the whole group, against the group's bounding box:
 //   - a node is skipped when even the nearest point of the box is beyond r_cut + sqrt(3) half
 //     (the reference's per-particle cube test, src/tree.rs:274-278, made conservative);
-//   - a walk-leaf contributes its particle range (direct sums);
+//   - a walk-leaf, or a node of at most kWalkLeaf particles, contributes its particle range
+//     (direct sums);
 //   - a split node is taken as a monopole only when 2 half < theta * d for the nearest point
 //     of the box, so the reference's criterion (src/tree.rs:297) holds for every particle of
 //     the group; otherwise its children are pushed.
@@ -51,10 +52,10 @@ struct WalkArgs {
     float eps2;
     float inv2rs;
     float negk;
-    // The S table is indexed by x = sqrt(r^2 + eps^2) * c (c = kSTab / rho_max): s' = (r^2 + eps^2) c^2
-    // gives x = s' rsqrt(s') and 1 / (r^2 + eps^2)^{3/2} = c^3 rsqrt(s')^3, so the table carries -K c^3.
-    float sc2;   // c^2
-    float tabk;  // -K c^3
+    // The S table is uniform in s = r^2 + eps^2 on [0, s_max]: x = s * sinv (sinv = kSTab / s_max)
+    // is the table coordinate; the table carries -K.
+    float sinv;
+    float tabk;  // -K
     const float2 *stab;
     uint32_t *next_group;
     const uint32_t *ngroups;  // device word written by the group builder
@@ -95,16 +96,27 @@ __device__ __forceinline__ void pair_f32(float dx, float dy, float dz, float r2,
     az = fmaf(dz, f, az);
 }
 
+// Exclusive prefix sum over the 64 lanes (all lanes must be active) with DPP row shifts and row
+// broadcasts, which move data between lanes without the LDS crossbar: Hillis-Steele within each
+// row of 16 lanes (shifts 1, 2, 4, 8), then lane 15 of each row is added to the next row and
+// lane 31 to the upper half. Lanes whose source is outside the row, or in a row the row mask
+// leaves out, receive 0.
 __device__ __forceinline__ int wave_excl_scan(int v, int &total) {
-    const int lane = threadIdx.x & 63;
     int x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    total = __shfl(x, 63, 64);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    total = __builtin_amdgcn_readlane(x, 63);
     return x - v;
+}
+
+// Position of this lane among the lanes whose bit is set in m (m from a ballot), and their count.
+__device__ __forceinline__ int mask_rank(uint64_t m, int &total) {
+    total = __popcll(m);
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 __device__ __forceinline__ double wave_min(double v) {
@@ -131,11 +143,18 @@ __device__ __forceinline__ double box_dist2(double dx, double dy, double dz, dou
 }
 
 constexpr int kWaves = 4;  // wavefronts per workgroup; each walks its own groups
-constexpr int kStackCap = 768;
+// Pops are limited so the stack stays below kStackCap + 7 * depth, within kStackAlloc for any tree.
+// A smaller cap costs more iterations only when the stack is nearly full; it leaves room in LDS
+// for a fifth workgroup per CU, which the walk's latency-bound traversal needs more.
+constexpr int kStackCap = 384;
 constexpr int kStackAlloc = kStackCap + 192;
 constexpr int kRangeCap = 128;
 constexpr int kNodeCap = 128;
 constexpr int kStage = 128;  // staged sources (kept ones accumulate here, evaluated 64 at a time)
+// Nodes of at most kWalkLeaf particles are listed as particle ranges without being opened: direct
+// sums over their particles (never less accurate than a monopole) instead of one more level of
+// traversal, whose dependent loads cost more than the extra candidates.
+constexpr uint32_t kWalkLeaf = 16;
 
 // Per-wavefront LDS state. The node tiles and the source stage are never live at the same time.
 struct WaveShared {
@@ -165,18 +184,18 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Pair force with -K c^3 S interpolated from the LDS table, linear in rho = sqrt(r^2 + eps^2) over
-// kSTab intervals of [0, rho_max], rho_max = sqrt((r_cut (1 + 1e-4))^2 + eps^2) (relative
-// interpolation error of S below 1e-5). One reciprocal square root gives both the table index
-// and the softened 1 / (r^2 + eps^2)^{3/2}. Used for particle pairs, which satisfy r <= r_cut, so
-// the interval index is at most kSTab - 1 and needs no clamp.
+// Pair force with -K S interpolated from the LDS table, linear in s = r^2 + eps^2 over kSTab
+// intervals of [0, s_max], s_max = (r_cut (1 + 1e-4))^2 + eps^2 (relative interpolation error of
+// the force below 5e-5, largest for the closest pairs). The table index does not depend on the
+// reciprocal square root, which only gives the softened 1 / (r^2 + eps^2)^{3/2}, so the two
+// chains overlap. Used for particle pairs, which satisfy r <= r_cut, so the interval index is at
+// most kSTab - 1 and needs no clamp.
 __device__ __forceinline__ void pair_tab(float dx, float dy, float dz, float s2, const float2 *stab,
                                          const WalkArgs &w, float &ax, float &ay, float &az) {
-    const float s = s2 * w.sc2;
-    const float rs = __builtin_amdgcn_rsqf(s);
-    const float x = s * rs;
+    const float x = s2 * w.sinv;
     const float fr = __builtin_amdgcn_fractf(x);
     const float2 t = stab[(int)x];
+    const float rs = __builtin_amdgcn_rsqf(s2);
     const float inv3 = rs * rs * rs;
     const float f = fmaf(fr, t.y, t.x) * inv3;
     ax = fmaf(dx, f, ax);
@@ -198,11 +217,12 @@ __device__ __forceinline__ void stage_one(const float4 src, const float2 *stab, 
     }
     // s2 = r^2 + eps^2; the cutoff r^2 <= r_cut^2 is tested as s2 <= r_cut^2 + eps^2.
     const float s2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, w.eps2)));
-    bool take = active && s2 <= w.cut2s;
+    // Inactive lanes hold NaN coordinates, so every comparison is false for them.
+    bool take = s2 <= w.cut2s;
     if (SELF) take = take && __float_as_uint(src.w) != self;
     if (take) {
         pair_tab(dx, dy, dz, s2, stab, w, ax, ay, az);
-        ++ninter;  // fills the wait state after the reciprocal square root: costs nothing
+        ++ninter;
     }
 }
 
@@ -335,7 +355,7 @@ __device__ __forceinline__ void eval_nodes(WaveShared &sh, int nn, const WalkArg
                 uy -= w.boxf * rintf(uy * w.invboxf);
                 uz -= w.boxf * rintf(uz * w.invboxf);
             }
-            const bool inside = active && fmaf(ux, ux, fmaf(uy, uy, uz * uz)) <= bb.w;
+            const bool inside = fmaf(ux, ux, fmaf(uy, uy, uz * uz)) <= bb.w;  // false for NaN (inactive) lanes
             if (!inside) continue;
             ++ninter;
             const float4 src = sh.tileA[s];
@@ -377,9 +397,12 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
         uint2 rng = make_uint2(0, 0);
         uint4 lk = make_uint4(0, 0, 0, 0);
         if (node != 0xFFFFFFFFu) {
+            // All four records are requested at once (the centre of mass is needed only for split
+            // nodes, but waiting for the link first would add a second round trip).
             lk = w.t.link[node];
             rng = w.t.range[node];
             const float4 bx = w.t.box[node];
+            const double4 cm = w.t.com[node];
             double dx = (double)bx.x - gx, dy = (double)bx.y - gy, dz = (double)bx.z - gz;
             if (!WRAP) {
                 dx = min_image64(dx, w.box, w.inv_box);
@@ -389,10 +412,9 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
             const double maxr = w.rcut + sqrt3 * (double)bx.w;
             const double d2 = box_dist2(dx, dy, dz, hx, hy, hz);
             if (WRAP || d2 <= maxr * maxr * (1.0 + 1e-6)) {
-                if (lk.w >> 8) {
+                if ((lk.w >> 8) || rng.y - rng.x <= kWalkLeaf) {
                     kind = 1;
                 } else {
-                    const double4 cm = w.t.com[node];
                     double cx = cm.x - gx, cy = cm.y - gy, cz = cm.z - gz;
                     if (!WRAP) {
                         cx = min_image64(cx, w.box, w.inv_box);
@@ -408,9 +430,9 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
         // Appends (wave-wide prefix sums).
         int t_open, t_leaf, t_leafcnt, t_acc;
         const int o_open = wave_excl_scan(kind == 3 ? (int)lk.y : 0, t_open);
-        const int o_leaf = wave_excl_scan(kind == 1 ? 1 : 0, t_leaf);
+        const int o_leaf = mask_rank(__builtin_amdgcn_ballot_w64(kind == 1), t_leaf);
         const int o_leafcnt = wave_excl_scan(kind == 1 ? (int)(rng.y - rng.x) : 0, t_leafcnt);
-        const int o_acc = wave_excl_scan(kind == 2 ? 1 : 0, t_acc);
+        const int o_acc = mask_rank(__builtin_amdgcn_ballot_w64(kind == 2), t_acc);
         if (kind == 3)
             for (uint32_t cix = 0; cix < lk.y; ++cix) {
                 const int at = size + o_open + (int)cix;
@@ -467,7 +489,10 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
     // Half extents, widened by a relative 1e-12 of the box so the box surely contains the group.
     const double pad = 1e-12 * w.box;
     const double hx = 0.5 * (mxx - mnx) + pad, hy = 0.5 * (mxy - mny) + pad, hz = 0.5 * (mxz - mnz) + pad;
-    const float xi = (float)(px - gx), yi = (float)(py - gy), zi = (float)(pz - gz);
+    // Inactive lanes get NaN coordinates: all their distance tests fail, so they take no pair.
+    const float nan = __builtin_nanf("");
+    const float xi = active ? (float)(px - gx) : nan, yi = active ? (float)(py - gy) : nan,
+                zi = active ? (float)(pz - gz) : nan;
     float ax = 0.f, ay = 0.f, az = 0.f;
     uint32_t ninter = 0, listed = 0, kept = 0;
     // Sources are placed at their image nearest the group centre; that image is the one nearest
@@ -691,9 +716,8 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     w.cut2s = w.cut2 + w.eps2;
     w.inv2rs = (float)(1.0 / (2.0 * c.p.r_split));
     w.negk = (float)(-k);
-    const double sc = c.stab_scale;
-    w.sc2 = (float)(sc * sc);
-    w.tabk = (float)(-k * sc * sc * sc);
+    w.sinv = (float)c.stab_scale;
+    w.tabk = (float)(-k);
     w.stab = c.stab;
     w.acc = d_acc;
     w.counter = c.counter;
@@ -731,8 +755,8 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
 }
 
 // S(u) = erfc(u) + (2/sqrt(pi)) u exp(-u^2) (src/tree.rs:54-57), u = r / (2 r_s), tabulated in FP64
-// against rho = sqrt(r^2 + eps^2) on [0, rho_max], rho_max = sqrt((r_cut (1 + 1e-4))^2 + eps^2):
-// kSTab + 1 points, value and slope per interval (entries below rho = eps, never read, hold S(0)).
+// against s = r^2 + eps^2 on [0, s_max], s_max = (r_cut (1 + 1e-4))^2 + eps^2: kSTab + 1 points,
+// value and slope per interval (entries below s = eps^2, never read, hold S(0)).
 void make_short_range_table(Ctx &c) {
     if (!(c.p.r_split > 0) || !(c.p.r_cut > 0)) {
         c.stab_scale = 0.0;
@@ -740,19 +764,19 @@ void make_short_range_table(Ctx &c) {
     }
     const double eps2 = c.p.softening * c.p.softening;
     const double rc = c.p.r_cut * (1.0 + 1e-4);
-    const double rho_max = std::sqrt(rc * rc + eps2);
-    const double drho = rho_max / kSTab;
-    auto S = [&](double rho) {
-        const double u = std::sqrt(std::max(rho * rho - eps2, 0.0)) / (2.0 * c.p.r_split);
+    const double s_max = rc * rc + eps2;
+    const double ds = s_max / kSTab;
+    auto S = [&](double s) {
+        const double u = std::sqrt(std::max(s - eps2, 0.0)) / (2.0 * c.p.r_split);
         return std::erfc(u) + 1.1283791670955126 * u * std::exp(-u * u);
     };
     std::vector<float2> t(kSTab + 1);
     for (int i = 0; i <= kSTab; ++i) {
-        const double v0 = S(i * drho), v1 = S((i + 1) * drho);
+        const double v0 = S(i * ds), v1 = S((i + 1) * ds);
         t[i] = make_float2((float)v0, (float)(v1 - v0));
     }
     TPM_HIP(hipMemcpy(c.stab, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
-    c.stab_scale = 1.0 / drho;
+    c.stab_scale = 1.0 / ds;
 }
 
 void walk_judge_f64(Ctx &c, size_t n, const uint32_t *d_targets_sorted, size_t m, double k, double theta,
