@@ -142,14 +142,16 @@ __device__ __forceinline__ double box_dist2(double dx, double dy, double dz, dou
     return ex * ex + ey * ey + ez * ez;
 }
 
-constexpr int kWaves = 4;  // wavefronts per workgroup; each walks its own groups
+// Wavefronts per workgroup; each walks its own groups. Eight share one copy of the S table, so
+// three workgroups (six wavefronts per SIMD) fit in the CU's LDS.
+constexpr int kWaves = 8;
 // Pops are limited so the stack stays below kStackCap + 7 * depth, within kStackAlloc for any tree.
 // A smaller cap costs more iterations only when the stack is nearly full; it leaves room in LDS
-// for a fifth workgroup per CU, which the walk's latency-bound traversal needs more.
-constexpr int kStackCap = 384;
+// for more resident wavefronts, which the walk's latency-bound traversal needs more.
+constexpr int kStackCap = 320;
 constexpr int kStackAlloc = kStackCap + 192;
-constexpr int kRangeCap = 128;
-constexpr int kNodeCap = 128;
+constexpr int kRangeCap = 96;
+constexpr int kNodeCap = 96;
 constexpr int kStage = 128;  // staged sources (kept ones accumulate here, evaluated 64 at a time)
 // Nodes of at most kWalkLeaf particles are listed as particle ranges without being opened: direct
 // sums over their particles (never less accurate than a monopole) instead of one more level of
@@ -162,6 +164,7 @@ struct WaveShared {
     uint32_t r_start[kRangeCap];
     uint32_t r_off[kRangeCap + 1];
     uint32_t nlist[kNodeCap];
+    uint32_t mark[64];  // range starts within a batch of 64 candidates, tagged by batch
     union {
         float4 stage[kStage];
         struct {
@@ -232,13 +235,13 @@ __device__ __forceinline__ void eval_stage(const WaveShared &sh, const float2 *s
                                            uint32_t self, float xi, float yi, float zi, bool active, float &ax,
                                            float &ay, float &az, uint32_t &ninter) {
     if (cnt == 64) {
-        // Eight sources are read ahead into registers so their LDS latency overlaps.
-        for (int c0 = 0; c0 < 64; c0 += 8) {
-            float4 src[8];
+        // Four sources are read ahead into registers so their LDS latency overlaps.
+        for (int c0 = 0; c0 < 64; c0 += 4) {
+            float4 src[4];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) src[j] = sh.stage[c0 + j];
+            for (int j = 0; j < 4; ++j) src[j] = sh.stage[c0 + j];
 #pragma unroll
-            for (int j = 0; j < 8; ++j)
+            for (int j = 0; j < 4; ++j)
                 stage_one<WRAP, SELF>(src[j], stab, w, self, xi, yi, zi, active, ax, ay, az, ninter);
         }
     } else {
@@ -252,51 +255,67 @@ __device__ __forceinline__ void eval_stage(const WaveShared &sh, const float2 *s
 // farther than r_cut from the whole box cannot be within r_cut of any particle of the group,
 // so it is dropped before the 64-lane evaluation. Kept sources are compacted into the stage
 // and evaluated 64 at a time.
+//
+// Candidate e of the listed ranges is found without a search: a batch of 64 candidates
+// [base, base + 64) lies in range rcur (the one holding candidate base) and the ranges that start
+// inside the batch, at most 64 after rcur. Those mark their start position in sh.mark with the
+// batch's tag; a ballot over the marks then gives, by a masked population count, the range of
+// every lane's candidate.
 template <bool WRAP, bool SELF>
 __device__ __forceinline__ void eval_particles(WaveShared &sh, const float2 *stab, int nr, int tot,
                                                const WalkArgs &w, double gx, double gy, double gz, float hx,
                                                float hy, float hz, uint32_t self, float xi, float yi, float zi,
                                                bool active, float &ax, float &ay, float &az, uint32_t &ninter,
-                                               uint32_t &kept) {
+                                               uint32_t &kept, uint32_t &tag) {
     const int lane = threadIdx.x & 63;
     if (lane == 0) sh.r_off[nr] = (uint32_t)tot;
     wave_sync();
     const float cut2 = w.cut2 * (1.0f + 1e-5f);
+    // |d| below this (float) bound cannot need a periodic shift: min_image(d) = d for |d| < L/2.
+    const float no_wrap = 0.49f * w.boxf;
     int cnt = 0;
-    for (int base = 0; base < tot; base += 64) {
+    int rcur = 0;
+    for (int base = 0; base < tot; base += 64, ++tag) {
+        const int r = rcur + 1 + lane;
+        if (r < nr) {
+            const int pos = (int)sh.r_off[r] - base;  // >= 0: range rcur holds candidate base
+            if (pos < 64) sh.mark[pos] = tag;
+        }
+        wave_sync();
+        const uint64_t starts = __builtin_amdgcn_ballot_w64(sh.mark[lane] == tag);
+        const int mine = rcur + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
+                         (int)((starts >> lane) & 1ull);
+        rcur += __popcll(starts);
         const int e = base + lane;
-        bool keep = false;
+        float d2 = __builtin_inff();
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (e < tot) {
-            int lo = 0, hi = nr - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if ((int)sh.r_off[mid] <= e)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            const uint32_t j = sh.r_start[lo] + (uint32_t)(e - (int)sh.r_off[lo]);
+            const uint32_t j = sh.r_start[mine] + (uint32_t)(e - (int)sh.r_off[mine]);
             double dx = w.spos[3 * (size_t)j] - gx;
             double dy = w.spos[3 * (size_t)j + 1] - gy;
             double dz = w.spos[3 * (size_t)j + 2] - gz;
-            if (!WRAP) {
-                dx = min_image64(dx, w.box, w.inv_box);
-                dy = min_image64(dy, w.box, w.inv_box);
-                dz = min_image64(dz, w.box, w.inv_box);
-            }
             v = make_float4((float)dx, (float)dy, (float)dz, __uint_as_float(j));
-            if (WRAP) {
-                keep = true;
-            } else {
+            if (!WRAP) {
+                const bool far = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))) >= no_wrap;
+                if (far) {
+                    dx = min_image64(dx, w.box, w.inv_box);
+                    dy = min_image64(dy, w.box, w.inv_box);
+                    dz = min_image64(dz, w.box, w.inv_box);
+                    v = make_float4((float)dx, (float)dy, (float)dz, v.w);
+                }
                 const float ex = fmaxf(fabsf(v.x) - hx, 0.f);
                 const float ey = fmaxf(fabsf(v.y) - hy, 0.f);
                 const float ez = fmaxf(fabsf(v.z) - hz, 0.f);
-                keep = fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= cut2;
+                d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+            } else {
+                d2 = 0.f;
             }
         }
-        const unsigned long long m = __ballot(keep);
-        if (keep) sh.stage[cnt + __popcll(m & ((1ull << lane) - 1ull))] = v;
+        const bool keep = d2 <= cut2;
+        const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
+        const int at = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (keep) sh.stage[cnt + at] = v;
         cnt += __popcll(m);
         wave_sync();
         if (cnt >= 64) {
@@ -377,7 +396,7 @@ template <bool WRAP, bool SELF>
 __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkArgs &w, double gx, double gy,
                                 double gz, double hx, double hy, double hz, uint32_t self, float xi, float yi,
                                 float zi, bool active, float &ax, float &ay, float &az, uint32_t &ninter,
-                                uint32_t &listed, uint32_t &kept) {
+                                uint32_t &listed, uint32_t &kept, uint32_t &tag) {
     const float hxf = (float)hx * (1.0f + 1e-6f), hyf = (float)hy * (1.0f + 1e-6f), hzf = (float)hz * (1.0f + 1e-6f);
     const int lane = threadIdx.x & 63;
     const double sqrt3 = 1.7320508075688772;
@@ -456,7 +475,7 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
         wave_sync();
         if (nr > kRangeCap - 64) {
             eval_particles<WRAP, SELF>(sh, stab, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active,
-                                       ax, ay, az, ninter, kept);
+                                       ax, ay, az, ninter, kept, tag);
             listed += (uint32_t)tot;
             nr = 0;
             tot = 0;
@@ -468,13 +487,21 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
     }
     if (nr > 0) {
         eval_particles<WRAP, SELF>(sh, stab, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active, ax,
-                                   ay, az, ninter, kept);
+                                   ay, az, ninter, kept, tag);
         listed += (uint32_t)tot;
     }
     if (nn > 0) eval_nodes<WRAP>(sh, nn, w, gx, gy, gz, xi, yi, zi, active, ax, ay, az, ninter);
 }
 
-__device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w, uint32_t g) {
+// Statistics of one wavefront's groups, summed in scalar registers and added to the global
+// counters once, when the wavefront exits (per-group atomics on three shared words serialise),
+// and the wavefront's candidate-batch tag (WaveShared::mark).
+struct WaveTotals {
+    unsigned long long inter = 0, listed = 0, kept = 0;
+    uint32_t tag = 0;
+};
+
+__device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w, uint32_t g, WaveTotals &tot) {
     const int lane = threadIdx.x & 63;
     const size_t g0 = w.gstart[g];
     const size_t g1 = w.gstart[g + 1];
@@ -507,17 +534,17 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
     if (wrap) {
         if (self_test)
             group_walk_body<true, true>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay, az,
-                                        ninter, listed, kept);
+                                        ninter, listed, kept, tot.tag);
         else
             group_walk_body<true, false>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay, az,
-                                         ninter, listed, kept);
+                                         ninter, listed, kept, tot.tag);
     } else {
         if (self_test)
             group_walk_body<false, true>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay,
-                                         az, ninter, listed, kept);
+                                         az, ninter, listed, kept, tot.tag);
         else
             group_walk_body<false, false>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay,
-                                          az, ninter, listed, kept);
+                                          az, ninter, listed, kept, tot.tag);
     }
     if (active) {
         const size_t o = w.idx[i];
@@ -527,11 +554,9 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
     }
     int ntot;
     wave_excl_scan(active ? (int)ninter : 0, ntot);
-    if (lane == 0 && w.counter) {
-        atomicAdd(w.counter, (unsigned long long)ntot);
-        atomicAdd(w.counter + 2, (unsigned long long)listed);
-        atomicAdd(w.counter + 3, (unsigned long long)kept);
-    }
+    tot.inter += (uint32_t)ntot;
+    tot.listed += listed;
+    tot.kept += kept;
 }
 
 // Persistent workgroups of kWaves wavefronts. The groups (Morton order) are split into
@@ -542,7 +567,8 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
 // the exit after visiting each range once and the load stays balanced.
 constexpr int kSlots = 8;
 
-__global__ void __launch_bounds__(64 * kWaves) k_group_walk(WalkArgs w) {
+// Six wavefronts per SIMD: the register budget is held to 80 VGPRs.
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(6, 8))) k_group_walk(WalkArgs w) {
     __shared__ BlockShared bs;
     // The table holds -K S(u) for this evaluation's coupling K.
     for (int i = threadIdx.x; i <= kSTab; i += blockDim.x) {
@@ -552,20 +578,28 @@ __global__ void __launch_bounds__(64 * kWaves) k_group_walk(WalkArgs w) {
     __syncthreads();
     WaveShared &sh = bs.wave[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
+    sh.mark[lane] = 0xFFFFFFFFu;  // never a tag
+    wave_sync();
     const uint32_t ng = *w.ngroups;
     int slot = (int)(blockIdx.x % kSlots);
+    WaveTotals tot;
     for (int visited = 0; visited < kSlots;) {
         const uint32_t r0 = (uint32_t)(((uint64_t)ng * slot) / kSlots);
         const uint32_t r1 = (uint32_t)(((uint64_t)ng * (slot + 1)) / kSlots);
         uint32_t g = 0;
         if (lane == 0) g = atomicAdd(w.next_group + slot, 1u);
-        g = __shfl(g, 0, 64);
+        g = __builtin_amdgcn_readfirstlane(g);
         if (g >= r1 - r0) {
             slot = (slot + 1) % kSlots;
             ++visited;
             continue;
         }
-        walk_group(sh, bs.stab, w, r0 + g);
+        walk_group(sh, bs.stab, w, r0 + g, tot);
+    }
+    if (lane == 0 && w.counter) {
+        atomicAdd(w.counter, tot.inter);
+        atomicAdd(w.counter + 2, tot.listed);
+        atomicAdd(w.counter + 3, tot.kept);
     }
 }
 
