@@ -119,16 +119,42 @@ __device__ __forceinline__ int mask_rank(uint64_t m, int &total) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// One DPP move of a double (both halves) with row control CTRL and row mask ROWS; lanes with no
+// source lane, or in rows the mask leaves out, receive `fill`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v, double fill) {
+    const uint64_t b = __double_as_longlong(v), f = __double_as_longlong(fill);
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)b, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Minimum and maximum over the 64 lanes, returned to every lane: the DPP steps of the prefix sum
+// (row shifts 1, 2, 4, 8, row broadcasts 15 and 31) leave the reduction in lane 63.
 __device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = fmin(v, __shfl_xor(v, d, 64));
-    return v;
+    const double inf = __builtin_inf();
+    v = fmin(v, dpp_f64<0x111, 0xf>(v, inf));
+    v = fmin(v, dpp_f64<0x112, 0xf>(v, inf));
+    v = fmin(v, dpp_f64<0x114, 0xf>(v, inf));
+    v = fmin(v, dpp_f64<0x118, 0xf>(v, inf));
+    v = fmin(v, dpp_f64<0x142, 0xa>(v, inf));
+    v = fmin(v, dpp_f64<0x143, 0xc>(v, inf));
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)b, 63), hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
-    return v;
+    const double ninf = -__builtin_inf();
+    v = fmax(v, dpp_f64<0x111, 0xf>(v, ninf));
+    v = fmax(v, dpp_f64<0x112, 0xf>(v, ninf));
+    v = fmax(v, dpp_f64<0x114, 0xf>(v, ninf));
+    v = fmax(v, dpp_f64<0x118, 0xf>(v, ninf));
+    v = fmax(v, dpp_f64<0x142, 0xa>(v, ninf));
+    v = fmax(v, dpp_f64<0x143, 0xc>(v, ninf));
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)b, 63), hi = __builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 __device__ __forceinline__ double min_image64(double d, double box, double inv_box) { return d - box * rint(d * inv_box); }
@@ -432,7 +458,11 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
             const double d2 = box_dist2(dx, dy, dz, hx, hy, hz);
             if (WRAP || d2 <= maxr * maxr * (1.0 + 1e-6)) {
                 if ((lk.w >> 8) || rng.y - rng.x <= kWalkLeaf) {
-                    kind = 1;
+                    // Listed only if the node's cube comes within r_cut of the group's box: no
+                    // particle farther away can be within r_cut of a member.
+                    const double hw = (double)bx.w;
+                    const double dc = box_dist2(dx, dy, dz, hx + hw, hy + hw, hz + hw);
+                    kind = (WRAP || dc <= w.rcut * w.rcut * (1.0 + 1e-6)) ? 1 : 0;
                 } else {
                     double cx = cm.x - gx, cy = cm.y - gy, cz = cm.z - gz;
                     if (!WRAP) {
