@@ -45,17 +45,21 @@ struct WalkArgs {
     double inv_box;
     double rcut;
     double theta2;
+    // The FP32 pair arithmetic works in coordinates scaled by c = sqrt(kSTab / s_max), where the S
+    // table, uniform in s = r^2 + eps^2 on [0, s_max], has unit spacing: s' = c^2 s is the table
+    // coordinate itself. All FP32 lengths below are scaled; forces come out unscaled because the
+    // couplings carry c^2 (f' dx' = K c^2 S / (c^3 rho^3) c dx = K S dx / rho^3).
+    double scale;  // c
     float boxf;
     float invboxf;
     float cut2;
     float cut2s;  // r_cut^2 + eps^2 (the pair cutoff on s2 = r^2 + eps^2)
     float eps2;
+    float rcutf;  // r_cut, scaled
+    float scalef;
     float inv2rs;
-    float negk;
-    // The S table is uniform in s = r^2 + eps^2 on [0, s_max]: x = s * sinv (sinv = kSTab / s_max)
-    // is the table coordinate; the table carries -K.
-    float sinv;
-    float tabk;  // -K
+    float negk;  // -K c^2
+    float tabk;  // -K c^2
     const float2 *stab;
     uint32_t *next_group;
     const uint32_t *ngroups;  // device word written by the group builder
@@ -221,7 +225,7 @@ __device__ __forceinline__ void wave_sync() {
 // most kSTab - 1 and needs no clamp.
 __device__ __forceinline__ void pair_tab(float dx, float dy, float dz, float s2, const float2 *stab,
                                          const WalkArgs &w, float &ax, float &ay, float &az) {
-    const float x = s2 * w.sinv;
+    const float x = s2;  // scaled: the table coordinate
     const float fr = __builtin_amdgcn_fractf(x);
     const float2 t = stab[(int)x];
     const float rs = __builtin_amdgcn_rsqf(s2);
@@ -299,6 +303,7 @@ __device__ __forceinline__ void eval_particles(WaveShared &sh, const float2 *sta
     const float cut2 = w.cut2 * (1.0f + 1e-5f);
     // |d| below this (float) bound cannot need a periodic shift: min_image(d) = d for |d| < L/2.
     const float no_wrap = 0.49f * w.boxf;
+    const double sc = w.scale;
     int cnt = 0;
     int rcur = 0;
     for (int base = 0; base < tot; base += 64, ++tag) {
@@ -321,14 +326,14 @@ __device__ __forceinline__ void eval_particles(WaveShared &sh, const float2 *sta
             double dx = w.spos[3 * (size_t)j] - gx;
             double dy = w.spos[3 * (size_t)j + 1] - gy;
             double dz = w.spos[3 * (size_t)j + 2] - gz;
-            v = make_float4((float)dx, (float)dy, (float)dz, __uint_as_float(j));
+            v = make_float4((float)(dx * sc), (float)(dy * sc), (float)(dz * sc), __uint_as_float(j));
             if (!WRAP) {
                 const bool far = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))) >= no_wrap;
                 if (far) {
                     dx = min_image64(dx, w.box, w.inv_box);
                     dy = min_image64(dy, w.box, w.inv_box);
                     dz = min_image64(dz, w.box, w.inv_box);
-                    v = make_float4((float)dx, (float)dy, (float)dz, v.w);
+                    v = make_float4((float)(dx * sc), (float)(dy * sc), (float)(dz * sc), v.w);
                 }
                 const float ex = fmaxf(fabsf(v.x) - hx, 0.f);
                 const float ey = fmaxf(fabsf(v.y) - hy, 0.f);
@@ -386,9 +391,10 @@ __device__ __forceinline__ void eval_nodes(WaveShared &sh, int nn, const WalkArg
                 bzz = min_image64(bzz, w.box, w.inv_box);
             }
             // src/tree.rs:275: max_r = r_cut + SQRT_3 * half, in FP32.
-            const float max_r = (float)w.rcut + sqrt3 * bx.w;
-            sh.tileA[lane] = make_float4((float)cx, (float)cy, (float)cz, (float)cm.w);
-            sh.tileB[lane] = make_float4((float)bxx, (float)byy, (float)bzz, max_r * max_r);
+            const float max_r = w.rcutf + sqrt3 * (bx.w * w.scalef);
+            const double sc = w.scale;
+            sh.tileA[lane] = make_float4((float)(cx * sc), (float)(cy * sc), (float)(cz * sc), (float)cm.w);
+            sh.tileB[lane] = make_float4((float)(bxx * sc), (float)(byy * sc), (float)(bzz * sc), max_r * max_r);
         }
         wave_sync();
         const int cnt = min(64, nn - base);
@@ -423,7 +429,8 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
                                 double gz, double hx, double hy, double hz, uint32_t self, float xi, float yi,
                                 float zi, bool active, float &ax, float &ay, float &az, uint32_t &ninter,
                                 uint32_t &listed, uint32_t &kept, uint32_t &tag) {
-    const float hxf = (float)hx * (1.0f + 1e-6f), hyf = (float)hy * (1.0f + 1e-6f), hzf = (float)hz * (1.0f + 1e-6f);
+    const float hxf = (float)(hx * w.scale) * (1.0f + 1e-6f), hyf = (float)(hy * w.scale) * (1.0f + 1e-6f),
+                hzf = (float)(hz * w.scale) * (1.0f + 1e-6f);
     const int lane = threadIdx.x & 63;
     const double sqrt3 = 1.7320508075688772;
     int size = 1;
@@ -548,8 +555,9 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
     const double hx = 0.5 * (mxx - mnx) + pad, hy = 0.5 * (mxy - mny) + pad, hz = 0.5 * (mxz - mnz) + pad;
     // Inactive lanes get NaN coordinates: all their distance tests fail, so they take no pair.
     const float nan = __builtin_nanf("");
-    const float xi = active ? (float)(px - gx) : nan, yi = active ? (float)(py - gy) : nan,
-                zi = active ? (float)(pz - gz) : nan;
+    const double sc = w.scale;
+    const float xi = active ? (float)((px - gx) * sc) : nan, yi = active ? (float)((py - gy) * sc) : nan,
+                zi = active ? (float)((pz - gz) * sc) : nan;
     float ax = 0.f, ay = 0.f, az = 0.f;
     uint32_t ninter = 0, listed = 0, kept = 0;
     // Sources are placed at their image nearest the group centre; that image is the one nearest
@@ -773,15 +781,19 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     w.inv_box = 1.0 / c.p.box_size;
     w.rcut = c.p.r_cut;
     w.theta2 = c.p.theta * c.p.theta;
-    w.boxf = (float)c.p.box_size;
-    w.invboxf = (float)(1.0 / c.p.box_size);
-    w.cut2 = (float)c.p.r_cut * (float)c.p.r_cut;
-    w.eps2 = (float)c.p.softening * (float)c.p.softening;
-    w.cut2s = w.cut2 + w.eps2;
-    w.inv2rs = (float)(1.0 / (2.0 * c.p.r_split));
-    w.negk = (float)(-k);
-    w.sinv = (float)c.stab_scale;
-    w.tabk = (float)(-k);
+    const double c2 = c.stab_scale > 0 ? c.stab_scale : 1.0;  // scale^2
+    const double sc = std::sqrt(c2);
+    w.scale = sc;
+    w.scalef = (float)sc;
+    w.boxf = (float)(c.p.box_size * sc);
+    w.invboxf = (float)(1.0 / (c.p.box_size * sc));
+    w.cut2 = (float)(c.p.r_cut * c.p.r_cut * c2);
+    w.eps2 = (float)(c.p.softening * c.p.softening * c2);
+    w.cut2s = (float)((c.p.r_cut * c.p.r_cut + c.p.softening * c.p.softening) * c2);
+    w.rcutf = (float)(c.p.r_cut * sc);
+    w.inv2rs = (float)(1.0 / (2.0 * c.p.r_split * sc));
+    w.negk = (float)(-k * c2);
+    w.tabk = (float)(-k * c2);
     w.stab = c.stab;
     w.acc = d_acc;
     w.counter = c.counter;
