@@ -164,7 +164,7 @@ def ensemble_bench(args):
     choices = {}
     for side, t in zip(sides, tuning):
         choices[str(side)] = {"fft_path": t["fft_path"], "fft_cap": t["fft_cap"], "tree_depth": t["tree_depth"],
-                              "group_level_shift": t["group_level_shift"],
+                              "group_level_shift": t["group_level_shift"], "small_node_direct": t["small_node_direct"],
                               "ms_default": round(t["ms_default"], 3), "ms_tuned": round(t["ms_tuned"], 3)}
     e.close()
     evals = total * (steps + 1)

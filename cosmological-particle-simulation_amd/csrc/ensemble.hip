@@ -68,6 +68,7 @@ static double tune_eval_ms(Ctx &c, const double *d_pos, size_t n, double a, floa
 tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_acc) {
     tpm_tuning t{};
     const int cap0 = c.fft_cap, shift0 = c.group_level_shift, path0 = c.fft_path, depth0 = c.tree_depth;
+    const int small0 = c.small_direct;
     double f0, w0;
     t.ms_default = tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f0, &w0);
     int best_cap = cap0, best_path = path0;
@@ -118,6 +119,13 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
         }
     }
     c.tree_depth = best_depth;
+    // Small nodes that pass the opening criterion: monopoles or direct sums (cheaper where such
+    // nodes are mostly near, as in nearly uniform boxes), judged on the whole evaluation.
+    int best_small = small0;
+    c.small_direct = 1 - small0;
+    if (tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr) < best_total) best_small = 1 - small0;
+    c.small_direct = best_small;
+    t.small_node_direct = best_small;
     t.tree_depth = best_depth;
     t.fft_cap = best_cap;
     t.fft_path = best_path;
@@ -129,6 +137,8 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
         c.fft_path = path0;
         c.group_level_shift = shift0;
         c.tree_depth = depth0;
+        c.small_direct = small0;
+        t.small_node_direct = small0;
         t.fft_cap = cap0;
         t.fft_path = path0;
         t.tree_depth = depth0;
@@ -254,6 +264,7 @@ int tpm_ensemble_autotune(tpm_ensemble *e, double a, tpm_tuning *out) {
                 c.fft_cap = bx.tuning.fft_cap;
                 c.fft_path = bx.tuning.fft_path;
                 c.tree_depth = bx.tuning.tree_depth;
+                c.small_direct = bx.tuning.small_node_direct;
                 c.group_level_shift = bx.tuning.group_level_shift;
             }
             if (out) out[b] = bx.tuning;
@@ -396,7 +407,9 @@ int tpm_set_tuning(tpm_ctx *ctx, const tpm_tuning *t) {
     Ctx &c = tpm_ctx_internal(ctx);
     if (t->fft_cap < 1 || t->fft_cap > 64 || t->fft_path < 0 || t->fft_path > 1) return TPM_E_INVALID;
     if (t->tree_depth < 10 || t->tree_depth > tpm::kMaxDepth) return TPM_E_INVALID;
+    if (t->small_node_direct < 0 || t->small_node_direct > 1) return TPM_E_INVALID;
     c.tree_depth = t->tree_depth;
+    c.small_direct = t->small_node_direct;
     c.fft_cap = t->fft_cap;
     c.fft_path = t->fft_path;
     c.group_level_shift = t->group_level_shift;

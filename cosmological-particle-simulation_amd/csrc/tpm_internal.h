@@ -155,6 +155,7 @@ struct Ctx {
     int group_level_shift = 0;  // autotuned offset of the group cube level
     int fft_cap = 16;           // autotuned cap on FFT lines per workgroup (LDS line kernels)
     int tree_depth = kMaxDepth; // octree depth limit (autotuned): deeper nodes stay leaves
+    int small_direct = 0;       // autotuned: small nodes passing the opening criterion summed directly
     int fft_path = 0;           // 0: wavefront transforms where the mesh side has them (512), 1: LDS line kernels
     uint32_t *tree_scan = nullptr;  // per-level child counts, offsets, scan partials, level table
     uint32_t *tree_info = nullptr;  // [0] node count, [1] capacity exceeded

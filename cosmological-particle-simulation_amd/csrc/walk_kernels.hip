@@ -6,8 +6,8 @@
 // wave walks the octree once for the whole group, against the group's bounding box:
 //   - a node is skipped when even the nearest point of the box is beyond r_cut + sqrt(3) half
 //     (the reference's per-particle cube test, src/tree.rs:274-278, made conservative);
-//   - a walk-leaf, or a node of at most kWalkLeaf particles, contributes its particle range
-//     (direct sums);
+//   - a walk-leaf, or a node of at most kWalkLeaf particles that would be opened, contributes its
+//     particle range (direct sums);
 //   - a split node is taken as a monopole only when 2 half < theta * d for the nearest point
 //     of the box, so the reference's criterion (src/tree.rs:297) holds for every particle of
 //     the group; otherwise its children are pushed.
@@ -45,6 +45,7 @@ struct WalkArgs {
     double inv_box;
     double rcut;
     double theta2;
+    int small_direct;  // nodes of at most kWalkLeaf particles are listed even where they could be accepted
     // The FP32 pair arithmetic works in coordinates scaled by c = sqrt(kSTab / s_max), where the S
     // table, uniform in s = r^2 + eps^2 on [0, s_max], has unit spacing: s' = c^2 s is the table
     // coordinate itself. All FP32 lengths below are scaled; forces come out unscaled because the
@@ -183,9 +184,9 @@ constexpr int kStackAlloc = kStackCap + 192;
 constexpr int kRangeCap = 96;
 constexpr int kNodeCap = 96;
 constexpr int kStage = 128;  // staged sources (kept ones accumulate here, evaluated 64 at a time)
-// Nodes of at most kWalkLeaf particles are listed as particle ranges without being opened: direct
-// sums over their particles (never less accurate than a monopole) instead of one more level of
-// traversal, whose dependent loads cost more than the extra candidates.
+// Nodes of at most kWalkLeaf particles that the opening criterion would open are listed as particle
+// ranges instead: direct sums over their particles rather than one more level of traversal, whose
+// dependent loads cost more than the extra candidates.
 constexpr uint32_t kWalkLeaf = 16;
 
 // Per-wavefront LDS state. The node tiles and the source stage are never live at the same time.
@@ -464,13 +465,8 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
             const double maxr = w.rcut + sqrt3 * (double)bx.w;
             const double d2 = box_dist2(dx, dy, dz, hx, hy, hz);
             if (WRAP || d2 <= maxr * maxr * (1.0 + 1e-6)) {
-                if ((lk.w >> 8) || rng.y - rng.x <= kWalkLeaf) {
-                    // Listed only if the node's cube comes within r_cut of the group's box: no
-                    // particle farther away can be within r_cut of a member.
-                    const double hw = (double)bx.w;
-                    const double dc = box_dist2(dx, dy, dz, hx + hw, hy + hw, hz + hw);
-                    kind = (WRAP || dc <= w.rcut * w.rcut * (1.0 + 1e-6)) ? 1 : 0;
-                } else {
+                bool leaf = (lk.w >> 8) != 0;
+                if (!leaf) {
                     double cx = cm.x - gx, cy = cm.y - gy, cz = cm.z - gz;
                     if (!WRAP) {
                         cx = min_image64(cx, w.box, w.inv_box);
@@ -479,7 +475,22 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
                     }
                     const double dc2 = WRAP ? 0.0 : box_dist2(cx, cy, cz, hx, hy, hz);
                     const double side = 2.0 * (double)bx.w;
-                    kind = (side * side < w.theta2 * dc2 * (1.0 - 1e-6)) ? 2 : 3;
+                    const bool small = rng.y - rng.x <= kWalkLeaf;
+                    if (small && w.small_direct)
+                        leaf = true;
+                    else if (side * side < w.theta2 * dc2 * (1.0 - 1e-6))
+                        kind = 2;
+                    else if (small)
+                        leaf = true;  // a small node that would be opened
+                    else
+                        kind = 3;
+                }
+                if (leaf) {
+                    // Listed only if the node's cube comes within r_cut of the group's box: no
+                    // particle farther away can be within r_cut of a member.
+                    const double hw = (double)bx.w;
+                    const double dc = box_dist2(dx, dy, dz, hx + hw, hy + hw, hz + hw);
+                    kind = (WRAP || dc <= w.rcut * w.rcut * (1.0 + 1e-6)) ? 1 : 0;
                 }
             }
         }
@@ -781,6 +792,7 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     w.inv_box = 1.0 / c.p.box_size;
     w.rcut = c.p.r_cut;
     w.theta2 = c.p.theta * c.p.theta;
+    w.small_direct = c.small_direct;
     const double c2 = c.stab_scale > 0 ? c.stab_scale : 1.0;  // scale^2
     const double sc = std::sqrt(c2);
     w.scale = sc;

@@ -128,7 +128,8 @@ class Stats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("fft_cap", ctypes.c_int32), ("group_level_shift", ctypes.c_int32), ("fft_path", ctypes.c_int32),
-                ("tree_depth", ctypes.c_int32), ("ms_default", ctypes.c_double), ("ms_tuned", ctypes.c_double)]
+                ("tree_depth", ctypes.c_int32), ("small_node_direct", ctypes.c_int32), ("ms_default", ctypes.c_double),
+                ("ms_tuned", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -519,10 +520,10 @@ class Solver:
                                           ctypes.byref(t)))
         return t
 
-    def set_tuning(self, fft_cap=16, group_level_shift=0, fft_path=0, tree_depth=21):
+    def set_tuning(self, fft_cap=16, group_level_shift=0, fft_path=0, tree_depth=21, small_node_direct=0):
         t = Tuning()
         t.fft_cap, t.group_level_shift, t.fft_path = int(fft_cap), int(group_level_shift), int(fft_path)
-        t.tree_depth = int(tree_depth)
+        t.tree_depth, t.small_node_direct = int(tree_depth), int(small_node_direct)
         self._check(self.lib.tpm_set_tuning(self.h, ctypes.byref(t)))
 
     def judge(self, pos, idx, a, theta=0.2):

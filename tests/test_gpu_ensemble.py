@@ -70,7 +70,7 @@ def test_autotune_keeps_results_and_is_not_slower(treepm):
     for t in choices:
         print(f"[gpu] tuning {t}", flush=True)
         assert t["fft_cap"] in (2, 4, 8, 16) and t["fft_path"] in (0, 1) and t["tree_depth"] in (12, 16, 21)
-        assert t["group_level_shift"] in (-1, 0, 1)
+        assert t["group_level_shift"] in (-1, 0, 1) and t["small_node_direct"] in (0, 1)
         assert t["ms_tuned"] <= t["ms_default"]
     e.run(A0, A1, 0, graph=True)
     got = _states(e)
@@ -124,8 +124,18 @@ def test_solver_autotune_and_set_tuning(treepm):
         e = np.sqrt(np.mean(np.sum((got - ref) ** 2, axis=1))) / scale
         print(f"[gpu] tree depth {depth}: force rms error {e:.3e}", flush=True)
         assert e < 1.05 * errs[0] + 1e-6
+    # small nodes summed directly where they could be monopoles: more direct sums, no less accurate
+    s.set_tuning(16, 0, small_node_direct=1)
+    s.accelerations_dev(d_pos.ptr, side ** 3, 0.5, d_acc.ptr)
+    s.sync()
+    got = d_acc.numpy().reshape(-1, 3)[idx].astype(np.float64)
+    e = np.sqrt(np.mean(np.sum((got - ref) ** 2, axis=1))) / scale
+    print(f"[gpu] small nodes summed directly: force rms error {e:.3e}", flush=True)
+    assert e < 1.05 * errs[0] + 1e-6
     with pytest.raises(treepm.TpmError):
         s.set_tuning(16, 0, tree_depth=9)
+    with pytest.raises(treepm.TpmError):
+        s.set_tuning(16, 0, small_node_direct=2)
     s.set_tuning(16, 0)
     t = s.autotune(d_pos.ptr, side ** 3, 0.5, d_acc.ptr)
     assert t.ms_tuned <= t.ms_default
