@@ -109,7 +109,7 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
     // stay leaves (more direct sums). Judged on the whole evaluation.
     int best_depth = depth0;
     double best_total = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr);
-    for (int depth : {16, 12}) {
+    for (int depth : {16, 12, 10}) {
         if (depth == depth0) continue;
         c.tree_depth = depth;
         const double tt = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr);
