@@ -106,9 +106,12 @@ def learned_vs_plain(args):
     s.sync()
     train_s = time.perf_counter() - t0
     log(f"[bench] learned: trained in {train_s:.1f} s, loss {list(hist)}")
-    # plain TreePM
+    # plain TreePM, with the knobs tuned on this input (the learned form runs with the same knobs)
     s.accelerations_dev(pos.ptr, npart, a, acc.ptr)
     s.sync()
+    tuning = s.autotune(pos.ptr, npart, a, acc.ptr).as_dict()
+    s.sync()
+    log(f"[bench] learned: plain TreePM tuning {tuning}")
     t0 = time.perf_counter()
     s.accelerations_dev(pos.ptr, npart, a, acc.ptr)
     s.sync()
@@ -128,7 +131,7 @@ def learned_vs_plain(args):
     return {"workload": f"treepm_nfw20000_{n}cubed_mesh{n}_z0", "train_samples": int(min(args.learned_samples, npart)),
             "epochs": args.learned_epochs, "train_s": train_s, "loss_history": [float(v) for v in hist],
             "tol": args.learned_tol, "fallback_fraction": nf / npart, "ms_plain": ms_plain,
-            "ms_learned": ms_learned, "speedup": ms_plain / ms_learned, "error_plain": err_plain,
+            "ms_learned": ms_learned, "speedup": ms_plain / ms_learned, "tuning": tuning, "error_plain": err_plain,
             "error_learned": err_learned}
 
 
