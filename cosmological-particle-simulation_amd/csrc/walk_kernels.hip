@@ -6,8 +6,9 @@
 // wave walks the octree once for the whole group, against the group's bounding box:
 //   - a node is skipped when even the nearest point of the box is beyond r_cut + sqrt(3) half
 //     (the reference's per-particle cube test, src/tree.rs:274-278, made conservative);
-//   - a walk-leaf, or a node of at most kWalkLeaf particles that would be opened, contributes its
-//     particle range (direct sums);
+//   - a walk-leaf, or a node of at most kWalkLeaf particles that would be opened (with the
+//     small_direct knob, also one that would be accepted), contributes its particle range
+//     (direct sums), if its cube comes within r_cut of the box;
 //   - a split node is taken as a monopole only when 2 half < theta * d for the nearest point
 //     of the box, so the reference's criterion (src/tree.rs:297) holds for every particle of
 //     the group; otherwise its children are pushed.
@@ -17,8 +18,9 @@
 // the node's children instead: the result is never less accurate than the reference's.
 //
 // Pair force (src/tree.rs:61-66): a = -K m S(r) d / (r^2 + eps^2)^{3/2},
-// S(r) = erfc(u) + (2/sqrt(pi)) u exp(-u^2), u = r / (2 r_s). In the FP32 walk, erfc(u) is
-// the Abramowitz & Stegun 7.1.26 rational approximation (Hastings),
+// S(r) = erfc(u) + (2/sqrt(pi)) u exp(-u^2), u = r / (2 r_s). In the FP32 walk, particle pairs
+// (r <= r_cut) interpolate S from an LDS table (pair_tab); for nodes, whose centre of mass may
+// lie beyond r_cut, erfc(u) is the Abramowitz & Stegun 7.1.26 rational approximation (Hastings),
 //   erfc(u) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-u^2),  t = 1 / (1 + p u),
 // with absolute error below 1.5e-7, which shares exp(-u^2) with the second term of S.
 //
