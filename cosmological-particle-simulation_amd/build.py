@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "libtpm.so")
 BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["tpm_api.hip", "mesh_kernels.hip", "fft_kernels.hip", "tree_kernels.hip", "walk_kernels.hip", "ic_kernels.hip", "kdk_kernels.hip", "mlp_kernels.hip", "ensemble.hip", "runner.hip"]
+SOURCES = ["tpm_api.hip", "mesh_kernels.hip", "fft_kernels.hip", "tree_kernels.hip", "walk_kernels.hip", "ic_kernels.hip", "kdk_kernels.hip", "mlp_kernels.hip", "ensemble.hip", "runner.hip", "snapshot_io.cpp"]
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
     # Packed FP32 instructions stay off: the compiler would otherwise form v_pk_* from float code.
@@ -29,7 +29,7 @@ PER_FILE = {"mesh_kernels.hip": ["-ffp-contract=off"]}
 
 
 def compile_one(src):
-    obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC, *COMMON, *PER_FILE.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

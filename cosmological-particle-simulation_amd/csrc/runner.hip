@@ -154,6 +154,20 @@ int tpm_headless_run(const tpm_run_settings *settings, tpm_frame_fn on_frame, vo
         rc = tpm_create(&p, np, &sim.h);
         if (rc != TPM_OK) return rc;
         tpm::Ctx &c = tpm_ctx_internal(sim.h);
+        // The run's parameters for binary snapshots of its frames.
+        tpm_snapshot_header snap{};
+        snap.n = np;
+        snap.box_size = box;
+        snap.omega_m = cosmo.omega_m;
+        snap.omega_l = cosmo.omega_l;
+        snap.omega_r = cosmo.omega_r;
+        snap.h0 = (double)67.4f;
+        snap.softening = p.softening;
+        snap.r_split = p.r_split;
+        snap.r_cut = p.r_cut;
+        snap.theta = p.theta;
+        snap.mesh_n = mesh_n;
+        snap.seed = s.seed;
         const auto t0 = std::chrono::steady_clock::now();
         // Simulation::new: the force at a_start (src/integrator.rs:54-79).
         tpm::evaluate(c, pos.as<double>(), np, a_start, false, acc.as<float>(), delta.as<float>());
@@ -191,6 +205,12 @@ int tpm_headless_run(const tpm_run_settings *settings, tpm_frame_fn on_frame, vo
                 f.d_rsd = rsd.as<float>();
                 f.d_delta = delta.as<float>();
                 f.mesh_n = mesh_n;
+                f.a = a;
+                f.d_state_pos = pos.as<double>();
+                f.d_state_mom = mom.as<double>();
+                snap.a = a;
+                snap.step = i;
+                f.snapshot = &snap;
                 if (on_frame(user, &f) != 0) break;
             }
         }

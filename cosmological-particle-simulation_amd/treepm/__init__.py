@@ -28,6 +28,7 @@ TPM_E_INVALID = -1
 TPM_E_DEVICE = -2
 TPM_E_CAPACITY = -3
 TPM_E_NONFINITE = -4
+TPM_E_IO = -5
 
 FLAG_TREEPM = 1
 FLAG_REF_LEAF_RULE = 2
@@ -48,6 +49,7 @@ EXPORTS = [
     "tpm_growth", "tpm_power_spectrum", "tpm_power_sigma", "tpm_zeldovich_bbks", "tpm_zeldovich_bbks_dev",
     "tpm_snapshot_dev", "tpm_run_settings_default", "tpm_run_settings_sanitized", "tpm_headless_run", "tpm_downsample_grid_dev", "tpm_autotune", "tpm_set_tuning", "tpm_ensemble_create", "tpm_ensemble_destroy", "tpm_ensemble_last_error",
     "tpm_ensemble_ics", "tpm_ensemble_autotune", "tpm_ensemble_run", "tpm_ensemble_get",
+    "tpm_snapshot_write", "tpm_snapshot_read",
 ]
 
 
@@ -163,11 +165,26 @@ class RunSettings(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SnapshotHeader(ctypes.Structure):
+    """tpm_snapshot_header: the 128-byte header of a binary snapshot file."""
+    _fields_ = [("magic", ctypes.c_char * 8), ("version", ctypes.c_uint32), ("header_bytes", ctypes.c_uint32),
+                ("n", ctypes.c_uint64), ("a", ctypes.c_double), ("box_size", ctypes.c_double),
+                ("omega_m", ctypes.c_double), ("omega_l", ctypes.c_double), ("omega_r", ctypes.c_double),
+                ("h0", ctypes.c_double), ("softening", ctypes.c_double), ("r_split", ctypes.c_double),
+                ("r_cut", ctypes.c_double), ("theta", ctypes.c_double), ("mesh_n", ctypes.c_int32),
+                ("step", ctypes.c_int32), ("seed", ctypes.c_uint64), ("reserved", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Frame(ctypes.Structure):
     _fields_ = [("step", ctypes.c_int32), ("steps", ctypes.c_int32), ("redshift", ctypes.c_double),
                 ("max_speed", ctypes.c_double), ("elapsed_s", ctypes.c_double), ("n", ctypes.c_size_t),
                 ("d_pos", ctypes.c_void_p), ("d_speed", ctypes.c_void_p), ("d_density", ctypes.c_void_p),
-                ("d_rsd", ctypes.c_void_p), ("d_delta", ctypes.c_void_p), ("mesh_n", ctypes.c_int32)]
+                ("d_rsd", ctypes.c_void_p), ("d_delta", ctypes.c_void_p), ("mesh_n", ctypes.c_int32),
+                ("a", ctypes.c_double), ("d_state_pos", ctypes.c_void_p), ("d_state_mom", ctypes.c_void_p),
+                ("snapshot", ctypes.POINTER(SnapshotHeader))]
 
 
 FRAME_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(Frame))
@@ -231,6 +248,8 @@ def load():
         "tpm_run_settings_default": (None, [ctypes.POINTER(RunSettings)]),
         "tpm_run_settings_sanitized": (ctypes.c_int, [ctypes.POINTER(RunSettings), ctypes.POINTER(RunSettings)]),
         "tpm_headless_run": (ctypes.c_int, [ctypes.POINTER(RunSettings), FRAME_FN, P, P, P, S]),
+        "tpm_snapshot_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(SnapshotHeader), P, P]),
+        "tpm_snapshot_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(SnapshotHeader), P, P, S]),
         "tpm_snapshot_dev": (ctypes.c_int, [P, P, P, S, D, D, P, P, P, P, P, P]),
         "tpm_downsample_grid_dev": (ctypes.c_int, [P, P, P, ctypes.POINTER(ctypes.c_int32), P]),
         "tpm_autotune": (ctypes.c_int, [P, P, S, D, P, ctypes.POINTER(Tuning)]),
@@ -622,6 +641,36 @@ def headless_run(settings, on_frame=None, want_state=False):
     if rc != TPM_OK:
         raise TpmError(rc, lib.tpm_last_error(None).decode())
     return (pos, mom) if want_state else None
+
+
+def snapshot_write(path, header, pos, mom):
+    """Writes a binary snapshot (tpm_snapshot_write): header (SnapshotHeader; n is taken from pos),
+    positions and momenta (n x 3 float64, caller order)."""
+    pos, mom = _f64(pos, (-1, 3)), _f64(mom, (-1, 3))
+    if pos.shape != mom.shape:
+        raise ValueError("positions and momenta differ in shape")
+    h = SnapshotHeader.from_buffer_copy(header)
+    h.n = len(pos)
+    rc = load().tpm_snapshot_write(os.fsencode(path), ctypes.byref(h), _ptr(pos), _ptr(mom))
+    if rc != TPM_OK:
+        raise TpmError(rc, f"cannot write snapshot {path}")
+
+
+def snapshot_read(path, arrays=True):
+    """Reads a binary snapshot: (header, pos, mom), or the header alone when arrays is False."""
+    lib = load()
+    h = SnapshotHeader()
+    rc = lib.tpm_snapshot_read(os.fsencode(path), ctypes.byref(h), None, None, 0)
+    if rc != TPM_OK:
+        raise TpmError(rc, f"cannot read snapshot {path}")
+    if not arrays:
+        return h
+    pos = np.empty((h.n, 3), np.float64)
+    mom = np.empty((h.n, 3), np.float64)
+    rc = lib.tpm_snapshot_read(os.fsencode(path), ctypes.byref(h), _ptr(pos), _ptr(mom), int(h.n))
+    if rc != TPM_OK:
+        raise TpmError(rc, f"cannot read snapshot {path}")
+    return h, pos, mom
 
 
 def coupling(omega_m, box_size, a, n):

@@ -34,6 +34,7 @@ extern "C" {
 #define TPM_E_DEVICE (-2)    /* HIP runtime error */
 #define TPM_E_CAPACITY (-3)  /* more particles than the context was created for */
 #define TPM_E_NONFINITE (-4) /* non-finite acceleration detected (TPM_FLAG_CHECK_FINITE) */
+#define TPM_E_IO (-5)        /* file cannot be opened, written or read, or is not a snapshot */
 
 /* Flags for tpm_params.flags. */
 #define TPM_FLAG_TREEPM 1u           /* add the tree short-range force (split = Some(..)); off = pure PM */
@@ -259,13 +260,42 @@ int tpm_snapshot_dev(tpm_ctx *ctx, const double *d_pos, const double *d_mom, siz
 int tpm_downsample_grid_dev(tpm_ctx *ctx, const float *d_delta, float *d_out, int32_t *side, void *stream);
 
 /*
+ * Binary snapshot (planned in the reference, plan.md:160-163: positions, velocities, a, box size
+ * and parameters, saved at chosen redshifts; the reference defines no format). File layout,
+ * little-endian: this 128-byte header (magic "TPMSNAP1", version 1), then n positions and n
+ * momenta, 3 doubles each, in caller order. The momentum is the reference's p (velocity = p / a).
+ * tpm_snapshot_write sets magic, version and header_bytes itself. tpm_snapshot_read reads the
+ * header and, when pos or mom is not NULL, the arrays (capacity >= n particles, else TPM_E_INVALID);
+ * TPM_E_IO for a file that cannot be read or is not a snapshot. Host memory.
+ */
+typedef struct tpm_snapshot_header {
+    char magic[8];
+    uint32_t version;
+    uint32_t header_bytes;
+    uint64_t n;
+    double a;
+    double box_size;
+    double omega_m, omega_l, omega_r;
+    double h0;        /* Hubble parameter of the ICs' power spectrum, km/s/Mpc */
+    double softening; /* force split (zeros for a PM-only run) */
+    double r_split;
+    double r_cut;
+    double theta;
+    int32_t mesh_n;
+    int32_t step; /* KDK steps taken */
+    uint64_t seed;
+    uint64_t reserved;
+} tpm_snapshot_header;
+
+/*
  * Headless runner: RunSettings (src/runner.rs:29-82) and the worker loop of the `--no-viz` run
  * (src/runner.rs:186-260, src/main.rs:34-37) on the GPU. tpm_run_settings_default gives
  * RunSettings::default, tpm_run_settings_sanitized applies RunSettings::sanitized's clamps (FP32).
  * tpm_headless_run sanitizes, builds the ICs (BBKS/sigma_8 Zel'dovich at z_start, seed), runs
  * `steps` KDK steps evenly spaced in ln a on a mesh of twice the particle count per side (TreePM
  * with ForceSplit::with when treepm != 0) and calls on_frame after each step with the snapshot in
- * device memory (valid during the call); a non-zero return stops the run. The final positions and
+ * device memory (valid during the call; `snapshot` describes the run for tpm_snapshot_write with
+ * d_state_pos and d_state_mom copied to the host); a non-zero return stops the run. The final positions and
  * momenta (3 N doubles each, N = particles_per_side^3 <= capacity) are copied out when given.
  */
 typedef struct tpm_run_settings {
@@ -293,6 +323,10 @@ typedef struct tpm_frame {
     const float *d_rsd;     /* 3N: p / (a^2 E) */
     const float *d_delta;   /* mesh_n^3 density contrast of the step's force evaluation */
     int32_t mesh_n;
+    double a;                  /* scale factor after the step */
+    const double *d_state_pos; /* 3N: the FP64 positions (device, valid during the callback) */
+    const double *d_state_mom; /* 3N: the FP64 momenta */
+    const tpm_snapshot_header *snapshot; /* this run's parameters, with a, n and step of this frame */
 } tpm_frame;
 
 typedef int (*tpm_frame_fn)(void *user, const tpm_frame *frame);
@@ -302,11 +336,15 @@ int tpm_run_settings_sanitized(const tpm_run_settings *in, tpm_run_settings *out
 int tpm_headless_run(const tpm_run_settings *settings, tpm_frame_fn on_frame, void *user, double *final_pos,
                      double *final_mom, size_t capacity);
 
+int tpm_snapshot_write(const char *path, const tpm_snapshot_header *header, const double *pos, const double *mom);
+int tpm_snapshot_read(const char *path, tpm_snapshot_header *header, double *pos, double *mom, size_t capacity);
+
 /*
- * Autotuning (no reference counterpart): times the candidates of three knobs on the given input and
- * keeps the fastest (the default is a candidate): the FFT kernels (wavefront transforms or LDS line
- * kernels), the cap on FFT lines per workgroup of the latter, the octree level of the walk
- * groups' cubes (as an offset from the default level) and the octree depth limit.
+ * Autotuning (no reference counterpart): times the candidates of the knobs below on the given input
+ * and keeps the fastest (the default is a candidate): the FFT kernels (wavefront transforms or LDS
+ * line kernels), the cap on FFT lines per workgroup of the latter, the octree level of the walk
+ * groups' cubes (as an offset from the default level), the octree depth limit and the small-node
+ * rule.
  */
 typedef struct tpm_tuning {
     int32_t fft_cap;
