@@ -191,7 +191,7 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
     interp_force(c, n, f32w, d_acc);
     tm.mark();
     if (treepm && n > 0) {
-        build_tree(c, n);
+        build_tree(c, n, !(c.p.flags & TPM_FLAG_REF_WALK));
         tm.mark();
         walk_short_range(c, n, coupling(c, a, n, f32w), d_acc);
         tm.mark();
@@ -453,7 +453,7 @@ static int short_range(tpm_ctx *ctx, const void *pos, size_t n, double k, float 
         tpm::zero_fill(c, c.counter, 8 * sizeof(unsigned long long));
         tpm::sort_particles(c, c.pos, n);
         tpm::zero_fill(c, c.acc, 3 * n * sizeof(float));
-        tpm::build_tree(c, n);
+        tpm::build_tree(c, n, !(c.p.flags & TPM_FLAG_REF_WALK));
         tpm::walk_short_range(c, n, k, c.acc);
         TPM_HIP(hipMemcpyAsync(acc, c.acc, 3 * n * sizeof(float), hipMemcpyDeviceToHost, c.stream));
         unsigned long long cnt = 0;
@@ -657,7 +657,7 @@ int tpm_mlp_train_dev(tpm_ctx *ctx, const double *d_pos, size_t n, double a, siz
         const double k = 1.5 * c.p.omega_m * c.p.box_size * c.p.box_size * c.p.box_size / (4.0 * M_PI * a * (double)n);
         pm_part(c, d_pos, n, a, c.acc);
         tpm::zero_fill(c, c.acc_sr, 3 * n * sizeof(float));
-        tpm::build_tree(c, n);
+        tpm::build_tree(c, n, !(c.p.flags & TPM_FLAG_REF_WALK));
         tpm::walk_short_range(c, n, k, c.acc_sr);
         const size_t ns = std::min(n_samples, n);
         // Evenly spaced particles of the Morton order (every particle equally likely: density weighted).
@@ -763,7 +763,7 @@ int tpm_accelerations_learned_dev(tpm_ctx *ctx, const double *d_pos, size_t n, d
         const size_t nrej = (size_t)last[0] + last[1];
         if (n_fallback) *n_fallback = nrej;
         if (nrej > 0) {
-            tpm::build_tree(c, n);
+            tpm::build_tree(c, n, !(c.p.flags & TPM_FLAG_REF_WALK));
             tpm::walk_short_range(c, n, k, d_acc, c.members, nrej);
         }
         tm.mark();

@@ -29,6 +29,9 @@ constexpr int kKeyBits = 21;
 // bounds the depth here at 21 levels, below which particles share a leaf).
 constexpr int kLeafSize = 8;
 constexpr int kMaxDepth = kKeyBits;
+// The group walk never opens a node of at most kWalkLeaf particles: it takes it as a monopole or
+// lists its particles. Children of such nodes are built only for the per-particle walks.
+constexpr uint32_t kWalkLeaf = 16;
 // Particles per walk group: one wavefront.
 constexpr int kGroup = 64;
 
@@ -195,7 +198,7 @@ void interp_force(Ctx &c, size_t n, bool f32_weights, float *d_acc);
 void interp_one(Ctx &c, const float *d_cells, const double *d_pos, size_t n, bool f32_weights, float *d_out);
 
 // Octree over the sorted particles (levels, ranges, boxes, centres of mass).
-void build_tree(Ctx &c, size_t n);
+void build_tree(Ctx &c, size_t n, bool group_walk_only = false);
 
 // Walk groups: runs of at most 64 sorted particles that never cross a cube of octree level
 // c.group_level (sets c.gstart, c.ngroups).

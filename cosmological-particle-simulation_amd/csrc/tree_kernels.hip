@@ -83,9 +83,12 @@ __device__ __forceinline__ uint32_t lower_digit(const uint64_t *keys, uint32_t l
 constexpr int kScanBlocks = 256;
 
 // Pass 1 over one level: decide which nodes split and count their non-empty octants.
+// Nodes of at most `keep_whole` particles that would split get no children (the group walk never
+// opens them) but keep their walk-leaf status from the split they would have.
 __global__ void __launch_bounds__(256) k_count_children(const uint64_t *__restrict__ keys, TreeView t,
                                                         const uint32_t *__restrict__ lv, int level, int max_depth,
-                                                        uint32_t ref_leaf_rule, uint32_t *__restrict__ nchild) {
+                                                        uint32_t ref_leaf_rule, uint32_t keep_whole,
+                                                        uint32_t *__restrict__ nchild) {
     const uint32_t lb = lv[level], le = lv[level + 1];
     for (uint32_t q = lb + blockIdx.x * blockDim.x + threadIdx.x; q < le; q += gridDim.x * blockDim.x) {
         const uint2 r = t.range[q];
@@ -104,6 +107,7 @@ __global__ void __launch_bounds__(256) k_count_children(const uint64_t *__restri
         }
         // Walk-leaf: no children, or (reference rule) no child in octant 0 (src/tree.rs:280).
         const uint32_t walk_leaf = (nc == 0 || (ref_leaf_rule && !(mask & 1u))) ? 1u : 0u;
+        if (cnt <= keep_whole) nc = 0;
         uint4 l = t.link[q];
         l.y = nc;
         l.z = mask;
@@ -312,7 +316,7 @@ static void ensure_tree(Ctx &c, size_t cap) {
 
 size_t tree_capacity(size_t n) { return std::max<size_t>(1024, 2 * n + 1024); }
 
-void build_tree(Ctx &c, size_t n) {
+void build_tree(Ctx &c, size_t n, bool group_walk_only) {
     Tree &t = c.tree;
     ensure_tree(c, tree_capacity(n));
     uint32_t *nch = c.tree_scan;
@@ -326,11 +330,12 @@ void build_tree(Ctx &c, size_t n) {
     hipLaunchKernelGGL(k_tree_root, dim3(1), dim3(64), 0, c.stream, t.view(), (uint32_t)n, half, lv, info);
     TPM_HIP(hipGetLastError());
     const uint32_t ref_rule = (c.p.flags & TPM_FLAG_REF_LEAF_RULE) ? 1u : 0u;
+    const uint32_t keep_whole = group_walk_only ? kWalkLeaf : 0u;
     // Node-parallel kernels stride over each level with a grid sized for n particles.
     const unsigned grid = std::max(1u, std::min(blocks(n, 256), 4096u));
     for (int level = 0; level <= c.tree_depth; ++level) {
         hipLaunchKernelGGL(k_count_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level,
-                           c.tree_depth, ref_rule, nch);
+                           c.tree_depth, ref_rule, keep_whole, nch);
         hipLaunchKernelGGL(k_scan_partials, dim3(kScanBlocks), dim3(256), 0, c.stream, nch, lv, level, part);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlocks), 0, c.stream, part, lv, level, (uint32_t)t.cap,
                            info);

@@ -189,7 +189,6 @@ constexpr int kStage = 128;  // staged sources (kept ones accumulate here, evalu
 // Nodes of at most kWalkLeaf particles that the opening criterion would open are listed as particle
 // ranges instead: direct sums over their particles rather than one more level of traversal, whose
 // dependent loads cost more than the extra candidates.
-constexpr uint32_t kWalkLeaf = 16;
 
 // Per-wavefront LDS state. The node tiles and the source stage are never live at the same time.
 struct WaveShared {
@@ -208,7 +207,7 @@ struct WaveShared {
 };
 
 struct BlockShared {
-    float2 stab[kSTab + 1];  // S(u) table: value and slope per interval
+    float2 stab[kSTab + 1];  // S table: intercept and slope per interval
     WaveShared wave[kWaves];
 };
 
@@ -222,18 +221,21 @@ __device__ __forceinline__ void wave_sync() {
 
 // Pair force with -K S interpolated from the LDS table, linear in s = r^2 + eps^2 over kSTab
 // intervals of [0, s_max], s_max = (r_cut (1 + 1e-4))^2 + eps^2 (relative interpolation error of
-// the force below 5e-5, largest for the closest pairs). The table index does not depend on the
-// reciprocal square root, which only gives the softened 1 / (r^2 + eps^2)^{3/2}, so the two
-// chains overlap. Used for particle pairs, which satisfy r <= r_cut, so the interval index is at
-// most kSTab - 1 and needs no clamp.
+// the force below 5e-5, largest for the closest pairs). Interval k holds the line through its end
+// points as intercept a_k and slope b_k in the table coordinate s', so no fractional part is
+// needed: the force factor (a_k + b_k s') / s'^{3/2} is (a_k rs^2 + b_k) rs with rs = 1/sqrt(s'),
+// one multiply-add between two products (the terms are at most about 5 times their sum, so the
+// rounding stays at the level of the FP32 coordinate). The table index
+// does not depend on the reciprocal square root, which only gives the softened
+// 1 / (r^2 + eps^2)^{3/2}, so the two chains overlap. Used for particle pairs, which satisfy
+// r <= r_cut, so the interval index is at most kSTab - 1 and needs no clamp.
 __device__ __forceinline__ void pair_tab(float dx, float dy, float dz, float s2, const float2 *stab,
                                          const WalkArgs &w, float &ax, float &ay, float &az) {
     const float x = s2;  // scaled: the table coordinate
-    const float fr = __builtin_amdgcn_fractf(x);
     const float2 t = stab[(int)x];
     const float rs = __builtin_amdgcn_rsqf(s2);
-    const float inv3 = rs * rs * rs;
-    const float f = fmaf(fr, t.y, t.x) * inv3;
+    // (a + b s') / s'^{3/2} = (a rs^2 + b) rs, with rs^2 = 1 / s'.
+    const float f = fmaf(t.x, rs * rs, t.y) * rs;
     ax = fmaf(dx, f, ax);
     ay = fmaf(dy, f, ay);
     az = fmaf(dz, f, az);
@@ -846,7 +848,8 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
 
 // S(u) = erfc(u) + (2/sqrt(pi)) u exp(-u^2) (src/tree.rs:54-57), u = r / (2 r_s), tabulated in FP64
 // against s = r^2 + eps^2 on [0, s_max], s_max = (r_cut (1 + 1e-4))^2 + eps^2: kSTab + 1 points,
-// value and slope per interval (entries below s = eps^2, never read, hold S(0)).
+// the line through each interval's end points as intercept and slope in the table coordinate
+// (entries below s = eps^2, never read, are flat at S(0)).
 void make_short_range_table(Ctx &c) {
     if (!(c.p.r_split > 0) || !(c.p.r_cut > 0)) {
         c.stab_scale = 0.0;
@@ -862,8 +865,9 @@ void make_short_range_table(Ctx &c) {
     };
     std::vector<float2> t(kSTab + 1);
     for (int i = 0; i <= kSTab; ++i) {
+        // Interval i spans [i, i + 1) in the table coordinate: S ~ (v0 - i (v1 - v0)) + s' (v1 - v0).
         const double v0 = S(i * ds), v1 = S((i + 1) * ds);
-        t[i] = make_float2((float)v0, (float)(v1 - v0));
+        t[i] = make_float2((float)(v0 - i * (v1 - v0)), (float)(v1 - v0));
     }
     TPM_HIP(hipMemcpy(c.stab, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
     c.stab_scale = 1.0 / ds;
