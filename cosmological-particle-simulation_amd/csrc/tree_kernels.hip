@@ -6,9 +6,10 @@
 // (LEAF_SIZE, src/tree.rs:12). Sorting all particles once by a 63-bit Morton key whose 3-bit
 // digits are exactly those octant numbers puts every octree node's particles in one contiguous
 // run, in the same octant order. The tree is then built level by level in parallel: a node
-// with more than 8 particles finds its children's runs by binary search on the key digit of
-// the next level. Node cubes use the reference's FP32 centre/half arithmetic (centre +- half/2),
-// and centres of mass are accumulated bottom-up in FP64.
+// with more than 8 particles finds its children's runs from the key digit of the next level
+// (binary searches, or for at most 64 particles a count of the digits). Node cubes use the
+// reference's FP32 centre/half arithmetic (centre +- half/2), and centres of mass are
+// accumulated bottom-up in FP64.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -76,6 +77,33 @@ __device__ __forceinline__ uint32_t lower_digit(const uint64_t *keys, uint32_t l
     return lo;
 }
 
+// Octant boundaries of the sorted keys [lo, hi) at `level`: b[o] = first index whose digit is >= o.
+// A range of at most 64 keys counts its digits in one pass of independent loads (a 7-bit count
+// per octant in one 64-bit word); a larger one uses binary searches.
+__device__ __forceinline__ void digit_bounds(const uint64_t *keys, uint32_t lo, uint32_t hi, int level,
+                                             uint32_t b[9]) {
+    b[0] = lo;
+    b[8] = hi;
+    if (hi - lo <= 64) {
+        const int sh = 3 * (kKeyBits - level);
+        uint64_t f = 0;
+        uint32_t j = lo;
+        for (; j + 4 <= hi; j += 4) {
+            const uint64_t k0 = keys[j], k1 = keys[j + 1], k2 = keys[j + 2], k3 = keys[j + 3];
+            f += (1ull << (7 * ((k0 >> sh) & 7u))) + (1ull << (7 * ((k1 >> sh) & 7u))) +
+                 (1ull << (7 * ((k2 >> sh) & 7u))) + (1ull << (7 * ((k3 >> sh) & 7u)));
+        }
+        for (; j < hi; ++j) f += 1ull << (7 * ((keys[j] >> sh) & 7u));
+        uint32_t at = lo;
+        for (int o = 1; o < 8; ++o) {
+            at += (uint32_t)((f >> (7 * (o - 1))) & 127u);
+            b[o] = at;
+        }
+    } else {
+        for (int o = 1; o < 8; ++o) b[o] = lower_digit(keys, b[o - 1], hi, level, o);
+    }
+}
+
 // The octree is built level by level entirely on the device (no host synchronisation, so the
 // build can be captured in a HIP graph): lv[l] is the first node of level l, lv[l + 1] its end;
 // each level's child counts are scanned by a fixed-grid three-phase scan and the next level's
@@ -96,9 +124,7 @@ __global__ void __launch_bounds__(256) k_count_children(const uint64_t *__restri
         uint32_t mask = 0, nc = 0;
         if (cnt > (uint32_t)kLeafSize && level < max_depth) {
             uint32_t b[9];
-            b[0] = r.x;
-            b[8] = r.y;
-            for (int o = 1; o < 8; ++o) b[o] = lower_digit(keys, b[o - 1], r.y, level + 1, o);
+            digit_bounds(keys, r.x, r.y, level + 1, b);
             for (int o = 0; o < 8; ++o)
                 if (b[o + 1] > b[o]) {
                     mask |= 1u << o;
@@ -207,9 +233,7 @@ __global__ void __launch_bounds__(256) k_make_children(const uint64_t *__restric
         }
         const uint2 r = t.range[q];
         uint32_t b[9];
-        b[0] = r.x;
-        b[8] = r.y;
-        for (int o = 1; o < 8; ++o) b[o] = lower_digit(keys, b[o - 1], r.y, level + 1, o);
+        digit_bounds(keys, r.x, r.y, level + 1, b);
         const uint64_t first = (uint64_t)le + offset[q - lb];
         if (first + l.y > cap) {
             // Over capacity (reported through info[1]): the node stays a leaf so the walk stays exact.
