@@ -349,6 +349,13 @@ int tpm_set_timing(tpm_ctx *ctx, int enabled) {
     });
 }
 
+int tpm_set_counting(tpm_ctx *ctx, int enabled) {
+    return guarded(ctx, [&](Ctx &c) {
+        c.count_pairs = enabled != 0;
+        return TPM_OK;
+    });
+}
+
 int tpm_get_stats(const tpm_ctx *ctx, tpm_stats *out) {
     if (!ctx || !out) return TPM_E_INVALID;
     *out = ctx->c.stats;

@@ -107,6 +107,7 @@ struct Ctx {
     size_t hs = 0;      // n * n * hp
     hipStream_t stream = nullptr;
     bool timing = false;
+    bool count_pairs = true;  // walk interactions counted (tpm_set_counting)
     std::string err;
     tpm_stats stats{};
 

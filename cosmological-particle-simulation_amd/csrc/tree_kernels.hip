@@ -270,11 +270,25 @@ __global__ void __launch_bounds__(256) k_com_level(const double *__restrict__ sp
         const uint2 r = t.range[q];
         double sx = 0, sy = 0, sz = 0;
         if (l.y == 0) {
-            for (uint32_t j = r.x; j < r.y; ++j) {
+            // Two independent partial sums per axis, so consecutive loads need not wait on each other.
+            double tx = 0, ty = 0, tz = 0;
+            uint32_t j = r.x;
+            for (; j + 2 <= r.y; j += 2) {
+                sx += spos[3 * (size_t)j];
+                sy += spos[3 * (size_t)j + 1];
+                sz += spos[3 * (size_t)j + 2];
+                tx += spos[3 * (size_t)j + 3];
+                ty += spos[3 * (size_t)j + 4];
+                tz += spos[3 * (size_t)j + 5];
+            }
+            if (j < r.y) {
                 sx += spos[3 * (size_t)j];
                 sy += spos[3 * (size_t)j + 1];
                 sz += spos[3 * (size_t)j + 2];
             }
+            sx += tx;
+            sy += ty;
+            sz += tz;
         } else {
             for (uint32_t cix = 0; cix < l.y; ++cix) {
                 const double4 cc = t.com[l.x + cix];

@@ -241,7 +241,7 @@ __device__ __forceinline__ void pair_tab(float dx, float dy, float dz, float s2,
     az = fmaf(dz, f, az);
 }
 
-template <bool WRAP, bool SELF>
+template <bool WRAP, bool SELF, bool CNT>
 __device__ __forceinline__ void stage_one(const float4 src, const float2 *stab, const WalkArgs &w, uint32_t self,
                                           float xi, float yi, float zi, bool active, float &ax, float &ay,
                                           float &az, uint32_t &ninter) {
@@ -260,12 +260,12 @@ __device__ __forceinline__ void stage_one(const float4 src, const float2 *stab, 
     if (SELF) take = take && __float_as_uint(src.w) != self;
     if (take) {
         pair_tab(dx, dy, dz, s2, stab, w, ax, ay, az);
-        ++ninter;
+        if (CNT) ++ninter;
     }
 }
 
 // Pairs of every active lane with the first `cnt` staged sources (broadcast LDS reads).
-template <bool WRAP, bool SELF>
+template <bool WRAP, bool SELF, bool CNT>
 __device__ __forceinline__ void eval_stage(const WaveShared &sh, const float2 *stab, int cnt, const WalkArgs &w,
                                            uint32_t self, float xi, float yi, float zi, bool active, float &ax,
                                            float &ay, float &az, uint32_t &ninter) {
@@ -277,11 +277,11 @@ __device__ __forceinline__ void eval_stage(const WaveShared &sh, const float2 *s
             for (int j = 0; j < 4; ++j) src[j] = sh.stage[c0 + j];
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-                stage_one<WRAP, SELF>(src[j], stab, w, self, xi, yi, zi, active, ax, ay, az, ninter);
+                stage_one<WRAP, SELF, CNT>(src[j], stab, w, self, xi, yi, zi, active, ax, ay, az, ninter);
         }
     } else {
         for (int s = 0; s < cnt; ++s)
-            stage_one<WRAP, SELF>(sh.stage[s], stab, w, self, xi, yi, zi, active, ax, ay, az, ninter);
+            stage_one<WRAP, SELF, CNT>(sh.stage[s], stab, w, self, xi, yi, zi, active, ax, ay, az, ninter);
     }
 }
 
@@ -296,7 +296,7 @@ __device__ __forceinline__ void eval_stage(const WaveShared &sh, const float2 *s
 // inside the batch, at most 64 after rcur. Those mark their start position in sh.mark with the
 // batch's tag; a ballot over the marks then gives, by a masked population count, the range of
 // every lane's candidate.
-template <bool WRAP, bool SELF>
+template <bool WRAP, bool SELF, bool CNT>
 __device__ __forceinline__ void eval_particles(WaveShared &sh, const float2 *stab, int nr, int tot,
                                                const WalkArgs &w, double gx, double gy, double gz, float hx,
                                                float hy, float hz, uint32_t self, float xi, float yi, float zi,
@@ -355,7 +355,7 @@ __device__ __forceinline__ void eval_particles(WaveShared &sh, const float2 *sta
         cnt += __popcll(m);
         wave_sync();
         if (cnt >= 64) {
-            eval_stage<WRAP, SELF>(sh, stab, 64, w, self, xi, yi, zi, active, ax, ay, az, ninter);
+            eval_stage<WRAP, SELF, CNT>(sh, stab, 64, w, self, xi, yi, zi, active, ax, ay, az, ninter);
             kept += 64;
             const int rem = cnt - 64;
             float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -367,7 +367,7 @@ __device__ __forceinline__ void eval_particles(WaveShared &sh, const float2 *sta
         }
     }
     if (cnt > 0) {
-        eval_stage<WRAP, SELF>(sh, stab, cnt, w, self, xi, yi, zi, active, ax, ay, az, ninter);
+        eval_stage<WRAP, SELF, CNT>(sh, stab, cnt, w, self, xi, yi, zi, active, ax, ay, az, ninter);
         kept += cnt;
         wave_sync();
     }
@@ -429,7 +429,7 @@ __device__ __forceinline__ void eval_nodes(WaveShared &sh, int nn, const WalkArg
     }
 }
 
-template <bool WRAP, bool SELF>
+template <bool WRAP, bool SELF, bool CNT>
 __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkArgs &w, double gx, double gy,
                                 double gz, double hx, double hy, double hz, uint32_t self, float xi, float yi,
                                 float zi, bool active, float &ax, float &ay, float &az, uint32_t &ninter,
@@ -526,7 +526,7 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
         nn += t_acc;
         wave_sync();
         if (nr > kRangeCap - 64) {
-            eval_particles<WRAP, SELF>(sh, stab, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active,
+            eval_particles<WRAP, SELF, CNT>(sh, stab, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active,
                                        ax, ay, az, ninter, kept, tag);
             listed += (uint32_t)tot;
             nr = 0;
@@ -538,7 +538,7 @@ __device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkAr
         }
     }
     if (nr > 0) {
-        eval_particles<WRAP, SELF>(sh, stab, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active, ax,
+        eval_particles<WRAP, SELF, CNT>(sh, stab, nr, tot, w, gx, gy, gz, hxf, hyf, hzf, self, xi, yi, zi, active, ax,
                                    ay, az, ninter, kept, tag);
         listed += (uint32_t)tot;
     }
@@ -553,6 +553,7 @@ struct WaveTotals {
     uint32_t tag = 0;
 };
 
+template <bool CNT>
 __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w, uint32_t g, WaveTotals &tot) {
     const int lane = threadIdx.x & 63;
     const size_t g0 = w.gstart[g];
@@ -586,17 +587,17 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
     const uint32_t self = (uint32_t)i;
     if (wrap) {
         if (self_test)
-            group_walk_body<true, true>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay, az,
+            group_walk_body<true, true, CNT>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay, az,
                                         ninter, listed, kept, tot.tag);
         else
-            group_walk_body<true, false>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay, az,
+            group_walk_body<true, false, CNT>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay, az,
                                          ninter, listed, kept, tot.tag);
     } else {
         if (self_test)
-            group_walk_body<false, true>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay,
+            group_walk_body<false, true, CNT>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay,
                                          az, ninter, listed, kept, tot.tag);
         else
-            group_walk_body<false, false>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay,
+            group_walk_body<false, false, CNT>(sh, stab, w, gx, gy, gz, hx, hy, hz, self, xi, yi, zi, active, ax, ay,
                                           az, ninter, listed, kept, tot.tag);
     }
     if (active) {
@@ -620,7 +621,9 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
 // the exit after visiting each range once and the load stays balanced.
 constexpr int kSlots = 8;
 
-// Six wavefronts per SIMD: the register budget is held to 80 VGPRs.
+// Six wavefronts per SIMD: the register budget is held to 80 VGPRs. CNT: count the interactions
+// (one vector instruction per pair); without it the interaction total is not reported.
+template <bool CNT>
 __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(6, 8))) k_group_walk(WalkArgs w) {
     __shared__ BlockShared bs;
     // The table holds -K S(u) for this evaluation's coupling K.
@@ -647,10 +650,10 @@ __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_e
             ++visited;
             continue;
         }
-        walk_group(sh, bs.stab, w, r0 + g, tot);
+        walk_group<CNT>(sh, bs.stab, w, r0 + g, tot);
     }
     if (lane == 0 && w.counter) {
-        atomicAdd(w.counter, tot.inter);
+        if (CNT) atomicAdd(w.counter, tot.inter);
         atomicAdd(w.counter + 2, tot.listed);
         atomicAdd(w.counter + 3, tot.kept);
     }
@@ -829,7 +832,7 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     zero_fill(c, w.next_group, kSlots * sizeof(uint32_t));
     if (c.walk_blocks == 0) {
         int per_cu = 0;
-        TPM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_walk, 64 * kWaves, 0));
+        TPM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_walk<true>, 64 * kWaves, 0));
         c.walk_blocks = (unsigned)std::max(1, per_cu) * (unsigned)c.num_cu;
     }
     // Persistent grid; workgroups without a group left exit at once. Never more workgroups than
@@ -841,7 +844,10 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
         if (!c.walk_ev[1]) TPM_HIP(hipEventCreate(&c.walk_ev[1]));
         TPM_HIP(hipEventRecord(c.walk_ev[0], c.stream));
     }
-    hipLaunchKernelGGL(k_group_walk, dim3(nblk), dim3(64 * kWaves), 0, c.stream, w);
+    if (c.count_pairs)
+        hipLaunchKernelGGL(k_group_walk<true>, dim3(nblk), dim3(64 * kWaves), 0, c.stream, w);
+    else
+        hipLaunchKernelGGL(k_group_walk<false>, dim3(nblk), dim3(64 * kWaves), 0, c.stream, w);
     TPM_HIP(hipGetLastError());
     if (c.timing) TPM_HIP(hipEventRecord(c.walk_ev[1], c.stream));
 }

@@ -113,6 +113,9 @@ int tpm_accelerations_dev_f64(tpm_ctx *ctx, const double *d_pos, size_t n, doubl
  * tpm_set_timing). */
 int tpm_get_stats(const tpm_ctx *ctx, tpm_stats *out);
 int tpm_set_timing(tpm_ctx *ctx, int enabled);
+/* Interaction counting in the group walk (no reference counterpart): on by default; off, the walk
+ * saves one vector instruction per particle pair and tpm_stats.interactions reads 0. */
+int tpm_set_counting(tpm_ctx *ctx, int enabled);
 
 /* Mesh::deposit (src/mesh.rs:62-69): cells (n_mesh^3) = CIC deposit of `mass` per particle,
  * accumulated in 2^-32 fixed point (bit-exact run to run). */

@@ -284,6 +284,28 @@ def test_force_evaluation_is_bit_exact_run_to_run(treepm):
     assert np.array_equal(d1.view(np.uint32), d2.view(np.uint32))
 
 
+def test_walk_without_counting_gives_identical_forces(treepm):
+    # The interaction counter is instrumentation only: switching it off changes no bit of the
+    # forces; the interaction total then reads 0 while the other walk statistics stay.
+    n = 32
+    pos = _zeldovich_config(treepm, n)
+    split = treepm.force_split(100.0, n, n, 1 / 30, 0.5)
+    s = solver(treepm, n, 100.0, split, max_particles=n ** 3)
+    s.set_timing(True)
+    a1, _ = s.accelerations(pos, 0.02)
+    st1 = s.stats()
+    s.set_counting(False)
+    a2, _ = s.accelerations(pos, 0.02)
+    st2 = s.stats()
+    s.set_counting(True)
+    a3, _ = s.accelerations(pos, 0.02)
+    st3 = s.stats()
+    assert np.array_equal(a1.view(np.uint32), a2.view(np.uint32))
+    assert np.array_equal(a1.view(np.uint32), a3.view(np.uint32))
+    assert st1["interactions"] > 0 and st2["interactions"] == 0 and st3["interactions"] == st1["interactions"]
+    assert st2["kept"] == st1["kept"] and st2["listed"] == st1["listed"]
+
+
 def test_judge_matches_fp64_oracle(treepm, oracle):
     n = 16
     pos = _zeldovich_config(treepm, n)
