@@ -367,8 +367,9 @@ __global__ void __launch_bounds__(256) k_fft_x_green(typename Cx<R>::T *spec, ty
 
 // ------------------------------------------------------------------------------ Z inverse
 
-// c2r of three components: A -> g[0], Cc -> g[1], Bz * (-i k_z) -> g[2]. Output grids are
-// real R values written as OutT (float for the force grids, double for the judge).
+// c2r of three components: A -> x, Cc -> y, Bz * (-i k_z) -> z, written as OutT: float force grids
+// as (x, y, z) per mesh point (the interpolation gathers one record per corner), double grids (the
+// judge and the ICs) as three planes g[comp * m + cell].
 template <typename R, typename OutT>
 __global__ void __launch_bounds__(256) k_fft_z_inv3(const typename Cx<R>::T *A, const typename Cx<R>::T *Cc,
                                                     const typename Cx<R>::T *Bz, OutT *g, size_t m, PlanArg pl,
@@ -414,11 +415,21 @@ __global__ void __launch_bounds__(256) k_fft_z_inv3(const typename Cx<R>::T *A, 
     }
     __syncthreads();
     C *res = fft_lds<R>(a, b, 3 * nb, ld, pl, tw, true);
-    for (int e = threadIdx.x; e < 3 * nb * n; e += blockDim.x) {
-        const int row = e / n, z = e - row * n;
-        const int comp = row / nb, li = row - comp * nb;
-        const size_t l = l0 + li;
-        if (l < nlines) g[comp * m + l * n + z] = (OutT)res[row * ld + z].x;
+    if (sizeof(OutT) == sizeof(float)) {
+        // Consecutive threads write consecutive floats of the (x, y, z) records.
+        for (int e = threadIdx.x; e < 3 * nb * n; e += blockDim.x) {
+            const int li = e / (3 * n), rem = e - li * 3 * n;
+            const int z = rem / 3, comp = rem - 3 * z;
+            const size_t l = l0 + li;
+            if (l < nlines) g[3 * (l * n + z) + comp] = (OutT)res[(comp * nb + li) * ld + z].x;
+        }
+    } else {
+        for (int e = threadIdx.x; e < 3 * nb * n; e += blockDim.x) {
+            const int row = e / n, z = e - row * n;
+            const int comp = row / nb, li = row - comp * nb;
+            const size_t l = l0 + li;
+            if (l < nlines) g[comp * m + l * n + z] = (OutT)res[row * ld + z].x;
+        }
     }
 }
 
@@ -646,37 +657,47 @@ __global__ void __launch_bounds__(64 * kCols) k_col(float2 *__restrict__ S0, flo
     tile_store<AXIS>(MODE == 2 ? S1 : S0, lds, outer, kz0, hp);
 }
 
-// z pass, c2r of three components: A -> g[0], Cc -> g[1], Bz (-i k_z) -> g[2]; one (line,
-// component) per wavefront, the upper half of each line by Hermitian completion.
+// z pass, c2r of three components: A -> x, Cc -> y, Bz (-i k_z) -> z, the upper half of each line
+// by Hermitian completion; one line per wavefront, written as (x, y, z) per mesh point so that the
+// interpolation gathers one 12-byte record per corner.
 __global__ void __launch_bounds__(256) k_z_inv3(const float2 *__restrict__ A, const float2 *__restrict__ Cc,
-                                                const float2 *__restrict__ Bz, float *__restrict__ g, size_t m,
+                                                const float2 *__restrict__ Bz, float *__restrict__ g,
                                                 const float2 *__restrict__ tw, const double *__restrict__ kvec,
                                                 size_t nlines) {
     __shared__ float2 lds[4 * kLP];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const size_t item = (size_t)xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
-    const size_t l = item / 3;
-    const int comp = (int)(item - 3 * l);
-    if (l >= nlines) return;
+    const size_t l = (size_t)xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
+    if (l >= nlines) return;  // whole wavefronts; no workgroup barrier follows
     Tw T;
     load_tw(T, tw, lane);
-    const float2 *S = (comp == 0 ? A : comp == 1 ? Cc : Bz) + l * (kN / 2 + 1);
-    float2 v[8];
+    float r[3][8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int i = lane + 64 * q;
-        const int kz = i <= kN / 2 ? i : kN - i;
-        float2 u = S[kz];
-        if (comp == 2) {
-            const float k = (float)kvec[kz];
-            u = f2(u.y * k, -u.x * k);
+    for (int comp = 0; comp < 3; ++comp) {
+        const float2 *S = (comp == 0 ? A : comp == 1 ? Cc : Bz) + l * (kN / 2 + 1);
+        float2 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = lane + 64 * q;
+            const int kz = i <= kN / 2 ? i : kN - i;
+            float2 u = S[kz];
+            if (comp == 2) {
+                const float k = (float)kvec[kz];
+                u = f2(u.y * k, -u.x * k);
+            }
+            v[q] = i <= kN / 2 ? u : f2(u.x, -u.y);
         }
-        v[q] = i <= kN / 2 ? u : f2(u.x, -u.y);
-    }
-    fft<true>(v, lds + wave * kLP, T);
-    float *out = g + comp * m + l * kN;
+        fft<true>(v, lds + wave * kLP, T);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) out[lane + 64 * j] = v[j].x;
+        for (int j = 0; j < 8; ++j) r[comp][j] = v[j].x;
+    }
+    float *out = g + 3 * (l * kN);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const size_t c = (size_t)(lane + 64 * j);
+        out[3 * c] = r[0][j];
+        out[3 * c + 1] = r[1][j];
+        out[3 * c + 2] = r[2][j];
+    }
 }
 
 }  // namespace w512
@@ -769,8 +790,8 @@ static void pm_pipeline(Ctx &c, const ZSrc<R> &src, typename Cx<R>::T *SA, typen
             hipLaunchKernelGGL((w512::k_col<0, 1>), gcol, bcol, 0, c.stream, SA, (float2 *)nullptr, tw, kv, hp, ntiles,
                                green);
             hipLaunchKernelGGL((w512::k_col<0, 2>), gcol, bcol, 0, c.stream, SB, SC, tw, kv, hp, ntiles, green);
-            hipLaunchKernelGGL(w512::k_z_inv3, dim3((unsigned)(3 * lines / 4)), dim3(256), 0, c.stream, SA, SC, SB,
-                               (float *)gout, c.m, tw, kv, lines);
+            hipLaunchKernelGGL(w512::k_z_inv3, dim3((unsigned)(lines / 4)), dim3(256), 0, c.stream, SA, SC, SB,
+                               (float *)gout, tw, kv, lines);
             TPM_HIP(hipGetLastError());
             return;
         }

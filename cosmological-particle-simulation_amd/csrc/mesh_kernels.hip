@@ -248,14 +248,18 @@ __global__ void __launch_bounds__(256) k_interp3(const double *__restrict__ pos,
 #pragma unroll
     for (int c = 0; c < 8; ++c) w[c] = (float)st.w[c];
     const size_t o = order ? order[i] : i;
+    // g holds (x, y, z) per mesh point: one record per corner.
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
 #pragma unroll
-    for (int comp = 0; comp < 3; ++comp) {
-        const float *gc = g + comp * m;
-        float s = 0.0f;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) s = s + gc[st.idx[c]] * w[c];
-        acc[3 * o + comp] = s;
+    for (int c = 0; c < 8; ++c) {
+        const float *gc = g + 3 * (size_t)st.idx[c];
+        sx = sx + gc[0] * w[c];
+        sy = sy + gc[1] * w[c];
+        sz = sz + gc[2] * w[c];
     }
+    acc[3 * o] = sx;
+    acc[3 * o + 1] = sy;
+    acc[3 * o + 2] = sz;
 }
 
 template <typename T>

@@ -247,7 +247,7 @@ __global__ void k_features(const double *__restrict__ spos, const uint32_t *__re
                     iy += iy < 0 ? n : 0;
                     iz += iz < 0 ? n : 0;
                     const size_t cell = ((size_t)ix * n + iy) * n + iz;
-                    for (int a = 0; a < 3; ++a) g[a] += (float)w * force[a * m + cell];
+                    for (int a = 0; a < 3; ++a) g[a] += (float)w * force[3 * cell + a];
                 }
         for (int a = 0; a < 3; ++a) x[q++] = to_bf16(slog(g[a] * inv_fscale));
     }

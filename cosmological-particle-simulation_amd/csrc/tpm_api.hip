@@ -431,10 +431,15 @@ int tpm_pm_accel(tpm_ctx *ctx, const float *delta, double a, double r_split, flo
         TPM_REQUIRE(r_split >= 0 && std::isfinite(r_split), "r_split must be >= 0");
         TPM_HIP(hipMemcpyAsync(c.delta, delta, c.m * sizeof(float), hipMemcpyHostToDevice, c.stream));
         tpm::pm_from_delta(c, c.delta, a, r_split);
-        TPM_HIP(hipMemcpyAsync(gx, c.force, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
-        TPM_HIP(hipMemcpyAsync(gy, c.force + c.m, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
-        TPM_HIP(hipMemcpyAsync(gz, c.force + 2 * c.m, c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        // The device grid holds (x, y, z) per mesh point; the caller gets three planes.
+        std::vector<float> g(3 * c.m);
+        TPM_HIP(hipMemcpyAsync(g.data(), c.force, 3 * c.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
         TPM_HIP(hipStreamSynchronize(c.stream));
+        for (size_t i = 0; i < c.m; ++i) {
+            gx[i] = g[3 * i];
+            gy[i] = g[3 * i + 1];
+            gz[i] = g[3 * i + 2];
+        }
         return TPM_OK;
     });
 }

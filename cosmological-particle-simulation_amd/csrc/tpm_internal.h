@@ -186,7 +186,7 @@ void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double
 void grid_to_float(Ctx &c, size_t n_particles, float *delta_out, float *mass_out);
 
 // PM force grids from the fixed-point grid (forward r2c fused with the density contrast,
-// Green's function, gradient and inverse passes). Writes c.force (gx, gy, gz).
+// Green's function, gradient and inverse passes). Writes c.force as (gx, gy, gz) per mesh point.
 void pm_from_grid(Ctx &c, size_t n_particles, double a, double r_split, float *delta_out);
 // PM force grids from a density-contrast grid given on the device.
 void pm_from_delta(Ctx &c, const float *d_delta, double a, double r_split);
