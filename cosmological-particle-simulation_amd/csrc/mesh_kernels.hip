@@ -132,11 +132,13 @@ __global__ void k_node_starts(const uint64_t *__restrict__ keys, size_t np, int 
     starts[k] = (uint32_t)lo;
 }
 
-// Work items per cube: ceil(count / kDepItem) (entry `nodes` is 0, so its exclusive scan is the total).
+// Extra work items per cube beyond its first: ceil(count / kDepItem) - 1, 0 for an empty cube
+// (entry `nodes` is 0, so its exclusive scan is the total).
 __global__ void k_node_items(const uint32_t *__restrict__ starts, uint32_t nodes, uint32_t *__restrict__ items) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k > nodes) return;
-    items[k] = k == nodes ? 0u : (starts[k + 1] - starts[k] + kDepItem - 1) / kDepItem;
+    const uint32_t cnt = k == nodes ? 0u : starts[k + 1] - starts[k];
+    items[k] = cnt > kDepItem ? (cnt - 1) / kDepItem : 0u;
 }
 
 __device__ __forceinline__ uint32_t morton_axis(uint32_t k, int axis, int level) {
@@ -151,19 +153,30 @@ __global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__
                                                       int level, int n, double box, double mass,
                                                       unsigned long long *__restrict__ grid) {
     __shared__ unsigned long long tile[kDepTile];
-    // Item b belongs to the cube k with item_start[k] <= b < item_start[k + 1].
+    // Workgroup b < nodes takes the first item of cube b; workgroup nodes + e takes extra item e of
+    // the dense cubes: extra item e belongs to the cube k with item_start[k] <= e < item_start[k + 1]
+    // (item_start: exclusive scan of the extra items per cube), as its item 1 + e - item_start[k].
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    if (b >= item_start[nodes]) return;
-    uint32_t lo_k = 0, hi_k = nodes;  // last k in [0, nodes) with item_start[k] <= b
-    while (lo_k < hi_k) {
-        const uint32_t mid = (lo_k + hi_k + 1) >> 1;
-        if (item_start[mid] <= b)
-            lo_k = mid;
-        else
-            hi_k = mid - 1;
+    uint32_t k, sub;
+    if (b < nodes) {
+        k = b;
+        sub = 0;
+        if (starts[k] == starts[k + 1]) return;
+    } else {
+        const uint32_t e = b - nodes;
+        if (e >= item_start[nodes]) return;
+        uint32_t lo_k = 0, hi_k = nodes;  // last k in [0, nodes) with item_start[k] <= e
+        while (lo_k < hi_k) {
+            const uint32_t mid = (lo_k + hi_k + 1) >> 1;
+            if (item_start[mid] <= e)
+                lo_k = mid;
+            else
+                hi_k = mid - 1;
+        }
+        k = lo_k;
+        sub = 1 + e - item_start[k];
     }
-    const uint32_t k = lo_k;
-    const uint32_t p0 = starts[k] + (b - item_start[k]) * kDepItem;
+    const uint32_t p0 = starts[k] + sub * kDepItem;
     const uint32_t p1 = min(starts[k + 1], p0 + kDepItem);
     const int tid = threadIdx.x;
     const T h = (T)box / (T)n;
@@ -214,14 +227,30 @@ __global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__
         }
     }
     __syncthreads();
+    // Tile cell c -> (ix, iy, iz) by multiply-high with per-workgroup reciprocals m = ceil(2^32 / d),
+    // exact here (c * (m d - 2^32) < c d < 2^32 for c < 13^3); the tile spans at most 13 cells per
+    // axis, so for meshes of 16 cells or more one conditional add or subtract wraps it.
+    const uint32_t d12 = (uint32_t)(dim[1] * dim[2]), d2 = (uint32_t)dim[2];
+    const uint32_t m12 = (uint32_t)((0x100000000ull + d12 - 1) / d12), m2 = (uint32_t)((0x100000000ull + d2 - 1) / d2);
+    const bool small_mesh = n < 16;
     for (int c = tid; c < cells; c += 256) {
         const unsigned long long v = tile[c];
         if (!v) continue;
-        const int iz = c % dim[2];
-        const int iy = (c / dim[2]) % dim[1];
-        const int ix = c / (dim[2] * dim[1]);
-        const int64_t gx = wrap_cell(lo[0] + ix, n), gy = wrap_cell(lo[1] + iy, n), gz = wrap_cell(lo[2] + iz, n);
-        atomicAdd(grid + (gx * n + gy) * n + gz, v);
+        const uint32_t ix = __umulhi((uint32_t)c, m12);
+        const uint32_t r = (uint32_t)c - ix * d12;
+        const uint32_t iy = __umulhi(r, m2);
+        const uint32_t iz = r - iy * d2;
+        int gx = lo[0] + (int)ix, gy = lo[1] + (int)iy, gz = lo[2] + (int)iz;
+        if (small_mesh) {
+            gx = (int)wrap_cell(gx, n);
+            gy = (int)wrap_cell(gy, n);
+            gz = (int)wrap_cell(gz, n);
+        } else {
+            gx += gx < 0 ? n : (gx >= n ? -n : 0);
+            gy += gy < 0 ? n : (gy >= n ? -n : 0);
+            gz += gz < 0 ? n : (gz >= n ? -n : 0);
+        }
+        atomicAdd(grid + (size_t)(gx * n + gy) * (size_t)n + (size_t)gz, v);
     }
 }
 
@@ -375,7 +404,7 @@ void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double
                            items);
         size_t bytes = c.dep_scan_bytes;
         TPM_HIP(rocprim_exclusive_scan_u32(c.dep_scan_tmp, bytes, items, item_start, nodes + 1, c.stream));
-        // At most one partial item per cube plus n / kDepItem full ones.
+        // One workgroup per cube for its first item, plus at most n / kDepItem extra items.
         const size_t grid_items = (size_t)nodes + n / kDepItem + 1;
         if (f32_weights)
             hipLaunchKernelGGL(k_deposit_cube<float>, dim3((unsigned)grid_items), dim3(256), 0, c.stream, pos_src,
