@@ -16,6 +16,8 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include "tpm_internal.h"
 
@@ -391,37 +393,47 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
 
 // ------------------------------------------------------------------------- walk groups
 
-__global__ void k_group_seg(const uint64_t *__restrict__ keys, size_t n, int shift, uint32_t *__restrict__ segval) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const bool start = i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift);
-    segval[i] = start ? (uint32_t)i : 0u;
-}
+// Segment starts (the first particle of each level-l cube) for the max-scan: i where the cube
+// changes, else 0; the scan then gives every particle the first index of its cube.
+struct GroupSegStart {
+    const uint64_t *keys;
+    int shift;
+    __device__ uint32_t operator()(uint32_t i) const {
+        return (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? i : 0u;
+    }
+};
 
-__global__ void k_group_chunk(const uint32_t *__restrict__ segstart, size_t n, uint32_t *__restrict__ flag) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    flag[i] = ((uint32_t)i - segstart[i]) % (uint32_t)kGroup == 0 ? 1u : 0u;
-}
+// Group starts: every kGroup-th particle of a cube, counted from the cube's first one.
+struct GroupChunkStart {
+    const uint32_t *segstart;
+    __device__ uint32_t operator()(uint32_t i) const { return (i - segstart[i]) % (uint32_t)kGroup == 0 ? 1u : 0u; }
+};
 
-__global__ void k_group_starts(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ gid, size_t n,
+__global__ void k_group_starts(const uint32_t *__restrict__ segstart, const uint32_t *__restrict__ gid, size_t n,
                                uint32_t *__restrict__ gstart, uint32_t *__restrict__ ngroups) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (flag[i]) gstart[gid[i]] = (uint32_t)i;
+    const uint32_t flag = ((uint32_t)i - segstart[i]) % (uint32_t)kGroup == 0 ? 1u : 0u;
+    if (flag) gstart[gid[i]] = (uint32_t)i;
     if (i == n - 1) {
-        gstart[gid[i] + flag[i]] = (uint32_t)n;
-        *ngroups = gid[i] + flag[i];
+        gstart[gid[i] + flag] = (uint32_t)n;
+        *ngroups = gid[i] + flag;
     }
 }
 
+using SegIter = rocprim::transform_iterator<rocprim::counting_iterator<uint32_t>, GroupSegStart, uint32_t>;
+using ChunkIter = rocprim::transform_iterator<rocprim::counting_iterator<uint32_t>, GroupChunkStart, uint32_t>;
+
 size_t group_scan_bytes(size_t max_n) {
-    size_t a = 0, b = 0;
+    size_t a = 0, b = 0, e = 0;
     uint32_t *p = nullptr;
-    if (rocprim::inclusive_scan(nullptr, a, p, p, max_n, rocprim::maximum<uint32_t>()) != hipSuccess ||
-        rocprim::exclusive_scan(nullptr, b, p, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess)
+    const SegIter si(rocprim::counting_iterator<uint32_t>(0), GroupSegStart{nullptr, 0});
+    const ChunkIter ci(rocprim::counting_iterator<uint32_t>(0), GroupChunkStart{nullptr});
+    if (rocprim::inclusive_scan(nullptr, a, si, p, max_n, rocprim::maximum<uint32_t>()) != hipSuccess ||
+        rocprim::exclusive_scan(nullptr, b, ci, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess ||
+        rocprim::exclusive_scan(nullptr, e, p, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess)
         throw Error{TPM_E_DEVICE, "scan size query failed"};
-    return std::max(a, b);
+    return std::max(a, std::max(b, e));
 }
 
 void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
@@ -437,17 +449,17 @@ void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
     level = std::max(2, std::min(kMaxDepth, level + c.group_level_shift));
     c.group_level = level;
     const int shift = 3 * (kKeyBits - level);
-    uint32_t *segval = c.gscratch;
-    uint32_t *tmp = c.gscratch + c.max_n;
-    hipLaunchKernelGGL(k_group_seg, dim3(blocks(n, 256)), dim3(256), 0, c.stream, keys, n, shift, segval);
-    TPM_HIP(hipGetLastError());
+    uint32_t *gid = c.gscratch;
+    uint32_t *segstart = c.gscratch + c.max_n;
+    // First index of each particle's cube (max-scan of the cube starts), then the group number of
+    // each particle (sum-scan of the group starts); both scans read their input through iterators.
     size_t bytes = c.gscan_bytes;
-    TPM_HIP(rocprim::inclusive_scan(c.gscan_tmp, bytes, segval, tmp, n, rocprim::maximum<uint32_t>(), c.stream));
-    hipLaunchKernelGGL(k_group_chunk, dim3(blocks(n, 256)), dim3(256), 0, c.stream, tmp, n, segval);
-    TPM_HIP(hipGetLastError());
+    const SegIter si(rocprim::counting_iterator<uint32_t>(0), GroupSegStart{keys, shift});
+    TPM_HIP(rocprim::inclusive_scan(c.gscan_tmp, bytes, si, segstart, n, rocprim::maximum<uint32_t>(), c.stream));
     bytes = c.gscan_bytes;
-    TPM_HIP(rocprim::exclusive_scan(c.gscan_tmp, bytes, segval, tmp, 0u, n, rocprim::plus<uint32_t>(), c.stream));
-    hipLaunchKernelGGL(k_group_starts, dim3(blocks(n, 256)), dim3(256), 0, c.stream, segval, tmp, n, c.gstart,
+    const ChunkIter ci(rocprim::counting_iterator<uint32_t>(0), GroupChunkStart{segstart});
+    TPM_HIP(rocprim::exclusive_scan(c.gscan_tmp, bytes, ci, gid, 0u, n, rocprim::plus<uint32_t>(), c.stream));
+    hipLaunchKernelGGL(k_group_starts, dim3(blocks(n, 256)), dim3(256), 0, c.stream, segstart, gid, n, c.gstart,
                        c.ngroups_dev);
     TPM_HIP(hipGetLastError());
 }
