@@ -110,7 +110,7 @@ __device__ __forceinline__ void digit_bounds(const uint64_t *keys, uint32_t lo, 
 // build can be captured in a HIP graph): lv[l] is the first node of level l, lv[l + 1] its end;
 // each level's child counts are scanned by a fixed-grid three-phase scan and the next level's
 // range is written to lv[l + 2].
-constexpr int kScanBlocks = 256;
+constexpr int kScanBlocks = 1024;
 
 // Pass 1 over one level: decide which nodes split and count their non-empty octants.
 // Nodes of at most `keep_whole` particles that would split get no children (the group walk never
@@ -372,8 +372,15 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
     const uint32_t ref_rule = (c.p.flags & TPM_FLAG_REF_LEAF_RULE) ? 1u : 0u;
     const uint32_t keep_whole = group_walk_only ? kWalkLeaf : 0u;
     // Node-parallel kernels stride over each level with a grid sized for n particles.
-    const unsigned grid = std::max(1u, std::min(blocks(n, 256), 4096u));
+    // A level holds at most min(8^level, n) nodes (each holds a particle); the node-parallel kernels
+    // get about one thread per possible node, so the deep levels' latency-bound loops (key loads,
+    // record loads) have their iterations in flight at once, and the shallow levels launch little.
+    auto level_grid = [&](int level) {
+        const double bound = std::min(std::pow(8.0, (double)level), (double)n);
+        return (unsigned)std::max(1.0, std::min(std::ceil(bound / 256.0), 65536.0));
+    };
     for (int level = 0; level <= c.tree_depth; ++level) {
+        const unsigned grid = level_grid(level);
         hipLaunchKernelGGL(k_count_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level,
                            c.tree_depth, ref_rule, keep_whole, nch);
         hipLaunchKernelGGL(k_scan_partials, dim3(kScanBlocks), dim3(256), 0, c.stream, nch, lv, level, part);
@@ -386,7 +393,7 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
     }
     // Centres of mass, deepest level first (levels past the deepest are empty ranges).
     for (int level = c.tree_depth + 1; level >= 0; --level) {
-        hipLaunchKernelGGL(k_com_level, dim3(grid), dim3(256), 0, c.stream, c.spos, t.view(), lv, level);
+        hipLaunchKernelGGL(k_com_level, dim3(level_grid(level)), dim3(256), 0, c.stream, c.spos, t.view(), lv, level);
         TPM_HIP(hipGetLastError());
     }
 }
