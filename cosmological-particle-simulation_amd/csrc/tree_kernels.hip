@@ -428,7 +428,52 @@ __global__ void k_group_starts(const uint32_t *__restrict__ segstart, const uint
     }
 }
 
+// First sorted index of every level-l cube (starts[cubes] = n): binary search on the sorted keys.
+__global__ void k_cube_starts(const uint64_t *__restrict__ keys, uint32_t n, int shift, uint32_t cubes,
+                              uint32_t *__restrict__ starts) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > cubes) return;
+    if (k == cubes) {
+        starts[k] = n;
+        return;
+    }
+    const uint64_t key = (uint64_t)k << shift;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    starts[k] = lo;
+}
+
+// Groups per cube: ceil(count / kGroup) (entry `cubes` is 0, so its exclusive scan is the total).
+struct CubeGroups {
+    const uint32_t *starts;
+    uint32_t cubes;
+    __device__ uint32_t operator()(uint32_t k) const {
+        return k < cubes ? (starts[k + 1] - starts[k] + (uint32_t)kGroup - 1) / (uint32_t)kGroup : 0u;
+    }
+};
+
+// Cube k's groups start every kGroup particles from its first; the last thread writes the end.
+__global__ void k_cube_gstart(const uint32_t *__restrict__ starts, const uint32_t *__restrict__ gbase, uint32_t cubes,
+                              uint32_t *__restrict__ gstart, uint32_t *__restrict__ ngroups) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cubes) return;
+    const uint32_t s0 = starts[k], s1 = starts[k + 1];
+    uint32_t g = gbase[k];
+    for (uint32_t p = s0; p < s1; p += (uint32_t)kGroup) gstart[g++] = p;
+    if (k == cubes - 1) {
+        gstart[gbase[cubes]] = starts[cubes];
+        *ngroups = gbase[cubes];
+    }
+}
+
 using SegIter = rocprim::transform_iterator<rocprim::counting_iterator<uint32_t>, GroupSegStart, uint32_t>;
+using CubeIter = rocprim::transform_iterator<rocprim::counting_iterator<uint32_t>, CubeGroups, uint32_t>;
 using ChunkIter = rocprim::transform_iterator<rocprim::counting_iterator<uint32_t>, GroupChunkStart, uint32_t>;
 
 size_t group_scan_bytes(size_t max_n) {
@@ -436,11 +481,14 @@ size_t group_scan_bytes(size_t max_n) {
     uint32_t *p = nullptr;
     const SegIter si(rocprim::counting_iterator<uint32_t>(0), GroupSegStart{nullptr, 0});
     const ChunkIter ci(rocprim::counting_iterator<uint32_t>(0), GroupChunkStart{nullptr});
+    const CubeIter ki(rocprim::counting_iterator<uint32_t>(0), CubeGroups{nullptr, 0});
+    size_t f = 0;
     if (rocprim::inclusive_scan(nullptr, a, si, p, max_n, rocprim::maximum<uint32_t>()) != hipSuccess ||
         rocprim::exclusive_scan(nullptr, b, ci, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess ||
-        rocprim::exclusive_scan(nullptr, e, p, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess)
+        rocprim::exclusive_scan(nullptr, e, p, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess ||
+        rocprim::exclusive_scan(nullptr, f, ki, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess)
         throw Error{TPM_E_DEVICE, "scan size query failed"};
-    return std::max(a, std::max(b, e));
+    return std::max(std::max(a, f), std::max(b, e));
 }
 
 void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
@@ -456,6 +504,24 @@ void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
     level = std::max(2, std::min(kMaxDepth, level + c.group_level_shift));
     c.group_level = level;
     const int shift = 3 * (kKeyBits - level);
+    const double cubes_d = std::pow(8.0, (double)level);
+    if (cubes_d + 1.0 <= (double)c.max_n) {
+        // Few cubes (about one per 64 or 512 particles): the first particle of every cube by binary
+        // search, the groups per cube, an exclusive scan over the cubes and the group starts.
+        const uint32_t cubes = (uint32_t)cubes_d;
+        uint32_t *starts = c.gscratch;
+        uint32_t *gbase = c.gscratch + c.max_n;
+        hipLaunchKernelGGL(k_cube_starts, dim3(blocks((size_t)cubes + 1, 256)), dim3(256), 0, c.stream, keys,
+                           (uint32_t)n, shift, cubes, starts);
+        size_t bytes = c.gscan_bytes;
+        const CubeIter ki(rocprim::counting_iterator<uint32_t>(0), CubeGroups{starts, cubes});
+        TPM_HIP(rocprim::exclusive_scan(c.gscan_tmp, bytes, ki, gbase, 0u, (size_t)cubes + 1, rocprim::plus<uint32_t>(),
+                                        c.stream));
+        hipLaunchKernelGGL(k_cube_gstart, dim3(blocks(cubes, 256)), dim3(256), 0, c.stream, starts, gbase, cubes,
+                           c.gstart, c.ngroups_dev);
+        TPM_HIP(hipGetLastError());
+        return;
+    }
     uint32_t *gid = c.gscratch;
     uint32_t *segstart = c.gscratch + c.max_n;
     // First index of each particle's cube (max-scan of the cube starts), then the group number of
