@@ -284,6 +284,26 @@ def test_force_evaluation_is_bit_exact_run_to_run(treepm):
     assert np.array_equal(d1.view(np.uint32), d2.view(np.uint32))
 
 
+def test_group_building_paths_agree(treepm):
+    # Walk groups come from per-cube starts when the cube count fits the context's scratch and from
+    # scans over all particles otherwise; both give the same groups, hence bit-identical forces.
+    # 16^3 particles with the group level raised by 2: 8^4 = 4096 cubes, so a context for exactly
+    # 4096 particles takes the scan path and one for 8192 the per-cube path.
+    n = 16
+    pos = _zeldovich_config(treepm, n)
+    split = treepm.force_split(100.0, n, n, 1 / 30, 0.5)
+    out = []
+    for cap in (n ** 3, 2 * n ** 3):
+        s = solver(treepm, n, 100.0, split, max_particles=cap)
+        s.set_tuning(16, 2, fft_path=0, tree_depth=21, small_node_direct=0)
+        s.set_timing(True)
+        a, _ = s.accelerations(pos, 0.02)
+        out.append((a, s.stats()["groups"]))
+        s.close()
+    assert out[0][1] == out[1][1] > 0
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+
+
 def test_walk_without_counting_gives_identical_forces(treepm):
     # The interaction counter is instrumentation only: switching it off changes no bit of the
     # forces; the interaction total then reads 0 while the other walk statistics stay.
