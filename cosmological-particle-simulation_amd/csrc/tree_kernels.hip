@@ -33,31 +33,50 @@ __device__ __forceinline__ uint64_t spread3(uint32_t v) {
     return x;
 }
 
-__global__ void k_morton(const double *__restrict__ pos, size_t n, double box, uint64_t *__restrict__ keys,
-                         uint32_t *__restrict__ idx) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    // floor(x / L * 2^21): the division comes first, so a position exactly on a dyadic plane
-    // of the box maps to the upper cell, like the reference's `p >= centre` octant test.
-    const double q0 = floor((pos[3 * i] / box) * (double)(1u << kKeyBits));
-    const double q1 = floor((pos[3 * i + 1] / box) * (double)(1u << kKeyBits));
-    const double q2 = floor((pos[3 * i + 2] / box) * (double)(1u << kKeyBits));
+// 63-bit Morton key of a position: floor(x / L * 2^21) per axis, the division first, so a position
+// exactly on a dyadic plane of the box maps to the upper cell, like the reference's `p >= centre`
+// octant test.
+__device__ __forceinline__ uint64_t morton_key(const double *p, double box) {
+    const double q0 = floor((p[0] / box) * (double)(1u << kKeyBits));
+    const double q1 = floor((p[1] / box) * (double)(1u << kKeyBits));
+    const double q2 = floor((p[2] / box) * (double)(1u << kKeyBits));
     const double qmax = (double)((1u << kKeyBits) - 1);
     const uint32_t a = (uint32_t)fmin(fmax(q0, 0.0), qmax);
     const uint32_t b = (uint32_t)fmin(fmax(q1, 0.0), qmax);
     const uint32_t c = (uint32_t)fmin(fmax(q2, 0.0), qmax);
-    keys[i] = spread3(a) | (spread3(b) << 1) | (spread3(c) << 2);
+    return spread3(a) | (spread3(b) << 1) | (spread3(c) << 2);
+}
+
+__global__ void k_morton(const double *__restrict__ pos, size_t n, double box, uint64_t *__restrict__ keys,
+                         uint32_t *__restrict__ idx) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = morton_key(pos + 3 * i, box);
     idx[i] = (uint32_t)i;
 }
 
-__global__ void k_gather(const double *__restrict__ pos, const uint32_t *__restrict__ idx, size_t n,
-                         double *__restrict__ spos) {
+// The top 30 bits of the key (octree levels 1-10), for a sort of 32-bit keys.
+constexpr int kHiShift = 3 * kKeyBits - 30;
+
+__global__ void k_morton_hi(const double *__restrict__ pos, size_t n, double box, uint32_t *__restrict__ keys,
+                            uint32_t *__restrict__ idx) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = (uint32_t)(morton_key(pos + 3 * i, box) >> kHiShift);
+    idx[i] = (uint32_t)i;
+}
+
+// Positions in sorted order; with `keys`, also the full keys of the sorted particles.
+__global__ void k_gather(const double *__restrict__ pos, const uint32_t *__restrict__ idx, size_t n, double box,
+                         double *__restrict__ spos, uint64_t *__restrict__ keys) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const size_t j = idx[i];
-    spos[3 * i] = pos[3 * j];
-    spos[3 * i + 1] = pos[3 * j + 1];
-    spos[3 * i + 2] = pos[3 * j + 2];
+    const double p[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+    spos[3 * i] = p[0];
+    spos[3 * i + 1] = p[1];
+    spos[3 * i + 2] = p[2];
+    if (keys) keys[i] = morton_key(p, box);
 }
 
 // ---------------------------------------------------------------------------- tree build
@@ -321,18 +340,35 @@ static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t -
 
 void sort_particles(Ctx &c, const double *d_pos, size_t n) {
     if (n == 0) return;
+    size_t bytes = c.sort_tmp_bytes;
+    // Only the key bits of the octree's levels are sorted (a shallower tree sorts fewer bits; the
+    // sort is stable, so the order below that depth is the input order).
+    if (c.tree_depth <= 10) {
+        // At most 30 key bits: 32-bit keys halve the key traffic of the radix passes; the gather
+        // recomputes the full keys of the sorted particles.
+        uint32_t *k32 = reinterpret_cast<uint32_t *>(c.keys), *k32_alt = reinterpret_cast<uint32_t *>(c.keys_alt);
+        hipLaunchKernelGGL(k_morton_hi, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_pos, n, c.p.box_size, k32,
+                           c.idx);
+        TPM_HIP(hipGetLastError());
+        rocprim::double_buffer<uint32_t> kb(k32, k32_alt);
+        rocprim::double_buffer<uint32_t> vb(c.idx, c.idx_alt);
+        TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 30 - 3 * c.tree_depth, 30, c.stream));
+        if (vb.current() != c.idx) std::swap(c.idx, c.idx_alt);
+        hipLaunchKernelGGL(k_gather, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.p.box_size,
+                           c.spos, c.keys);
+        TPM_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_pos, n, c.p.box_size, c.keys, c.idx);
     TPM_HIP(hipGetLastError());
     rocprim::double_buffer<uint64_t> kb(c.keys, c.keys_alt);
     rocprim::double_buffer<uint32_t> vb(c.idx, c.idx_alt);
-    size_t bytes = c.sort_tmp_bytes;
-    // Only the key bits of the octree's levels are sorted (a shallower tree sorts fewer bits; the
-    // sort is stable, so the order below that depth is the input order).
     TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 3 * (kKeyBits - c.tree_depth), 3 * kKeyBits,
                                       c.stream));
     if (kb.current() != c.keys) std::swap(c.keys, c.keys_alt);
     if (vb.current() != c.idx) std::swap(c.idx, c.idx_alt);
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.spos);
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.p.box_size, c.spos,
+                       (uint64_t *)nullptr);
     TPM_HIP(hipGetLastError());
 }
 
@@ -543,12 +579,14 @@ hipError_t rocprim_exclusive_scan_u32(void *tmp, size_t &bytes, const uint32_t *
 }
 
 size_t sort_temp_bytes(size_t max_n) {
-    size_t bytes = 0;
+    size_t bytes = 0, bytes32 = 0;
     rocprim::double_buffer<uint64_t> kb(nullptr, nullptr);
+    rocprim::double_buffer<uint32_t> kb32(nullptr, nullptr);
     rocprim::double_buffer<uint32_t> vb(nullptr, nullptr);
-    hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, kb, vb, max_n, 0, 3 * kKeyBits);
-    if (e != hipSuccess) throw Error{TPM_E_DEVICE, "radix_sort_pairs size query failed"};
-    return bytes;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, kb, vb, max_n, 0, 3 * kKeyBits) != hipSuccess ||
+        rocprim::radix_sort_pairs(nullptr, bytes32, kb32, vb, max_n, 0, 30) != hipSuccess)
+        throw Error{TPM_E_DEVICE, "radix_sort_pairs size query failed"};
+    return std::max(bytes, bytes32);
 }
 
 }  // namespace tpm
