@@ -134,9 +134,20 @@ constexpr int kScanBlocks = 1024;
 // Pass 1 over one level: decide which nodes split and count their non-empty octants.
 // Nodes of at most `keep_whole` particles that would split get no children (the group walk never
 // opens them) but keep their walk-leaf status from the split they would have.
+// Octant boundaries of a node of `level` (cube prefix of its first key) from the table of the first
+// sorted index of every level-`tab_level` cube, for level < tab_level: no search at all.
+__device__ __forceinline__ void table_bounds(const uint64_t *keys, const uint32_t *tab, int tab_level, uint32_t first,
+                                             int level, uint32_t b[9]) {
+    const uint64_t cube = keys[first] >> (3 * (kKeyBits - level));
+    const int sh = 3 * (tab_level - level - 1);
+#pragma unroll
+    for (int o = 0; o <= 8; ++o) b[o] = tab[(size_t)((cube << 3) + (uint64_t)o) << sh];
+}
+
 __global__ void __launch_bounds__(256) k_count_children(const uint64_t *__restrict__ keys, TreeView t,
                                                         const uint32_t *__restrict__ lv, int level, int max_depth,
                                                         uint32_t ref_leaf_rule, uint32_t keep_whole,
+                                                        const uint32_t *__restrict__ tab, int tab_level,
                                                         uint32_t *__restrict__ nchild) {
     const uint32_t lb = lv[level], le = lv[level + 1];
     for (uint32_t q = lb + blockIdx.x * blockDim.x + threadIdx.x; q < le; q += gridDim.x * blockDim.x) {
@@ -145,7 +156,10 @@ __global__ void __launch_bounds__(256) k_count_children(const uint64_t *__restri
         uint32_t mask = 0, nc = 0;
         if (cnt > (uint32_t)kLeafSize && level < max_depth) {
             uint32_t b[9];
-            digit_bounds(keys, r.x, r.y, level + 1, b);
+            if (level < tab_level)
+                table_bounds(keys, tab, tab_level, r.x, level, b);
+            else
+                digit_bounds(keys, r.x, r.y, level + 1, b);
             for (int o = 0; o < 8; ++o)
                 if (b[o + 1] > b[o]) {
                     mask |= 1u << o;
@@ -243,7 +257,8 @@ __global__ void __launch_bounds__(256) k_scan_finish(const uint32_t *__restrict_
 // Pass 2: create the children of each split node at first = le + offset[q - lb].
 __global__ void __launch_bounds__(256) k_make_children(const uint64_t *__restrict__ keys, TreeView t,
                                                        const uint32_t *__restrict__ lv, int level,
-                                                       const uint32_t *__restrict__ offset, uint32_t cap) {
+                                                       const uint32_t *__restrict__ offset, uint32_t cap,
+                                                       const uint32_t *__restrict__ tab, int tab_level) {
     const uint32_t lb = lv[level], le = lv[level + 1];
     for (uint32_t q = lb + blockIdx.x * blockDim.x + threadIdx.x; q < le; q += gridDim.x * blockDim.x) {
         uint4 l = t.link[q];
@@ -254,7 +269,10 @@ __global__ void __launch_bounds__(256) k_make_children(const uint64_t *__restric
         }
         const uint2 r = t.range[q];
         uint32_t b[9];
-        digit_bounds(keys, r.x, r.y, level + 1, b);
+        if (level < tab_level)
+            table_bounds(keys, tab, tab_level, r.x, level, b);
+        else
+            digit_bounds(keys, r.x, r.y, level + 1, b);
         const uint64_t first = (uint64_t)le + offset[q - lb];
         if (first + l.y > cap) {
             // Over capacity (reported through info[1]): the node stays a leaf so the walk stays exact.
@@ -372,6 +390,9 @@ void sort_particles(Ctx &c, const double *d_pos, size_t n) {
     TPM_HIP(hipGetLastError());
 }
 
+__global__ void k_cube_starts(const uint64_t *__restrict__ keys, uint32_t n, int shift, uint32_t cubes,
+                              uint32_t *__restrict__ starts);
+
 // Tree arrays for up to `cap` nodes (allocated once per capacity; never resized during a build).
 static void ensure_tree(Ctx &c, size_t cap) {
     Tree &t = c.tree;
@@ -386,7 +407,8 @@ static void ensure_tree(Ctx &c, size_t cap) {
     TPM_HIP(hipMalloc(&t.box, cap * sizeof(float4)));
     TPM_HIP(hipMalloc(&t.com, cap * sizeof(double4)));
     // child counts and offsets (cap each), scan partials, level table and info words
-    TPM_HIP(hipMalloc(&c.tree_scan, (2 * cap + kScanBlocks + kMaxDepth + 8) * sizeof(uint32_t)));
+    // child counts and offsets (cap each), scan partials, level table, info words, cube table (< cap)
+    TPM_HIP(hipMalloc(&c.tree_scan, (3 * cap + kScanBlocks + kMaxDepth + 8) * sizeof(uint32_t)));
     t.cap = cap;
 }
 
@@ -407,7 +429,18 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
     TPM_HIP(hipGetLastError());
     const uint32_t ref_rule = (c.p.flags & TPM_FLAG_REF_LEAF_RULE) ? 1u : 0u;
     const uint32_t keep_whole = group_walk_only ? kWalkLeaf : 0u;
-    // Node-parallel kernels stride over each level with a grid sized for n particles.
+    // The upper levels take their octant boundaries from the first sorted index of every cube of
+    // level T, the deepest level with at most n / 16 cubes (one binary search per cube, all in
+    // parallel), instead of searching per node.
+    int tab_level = 0;
+    while (tab_level < c.tree_depth && std::pow(8.0, (double)(tab_level + 1)) * 16.0 <= (double)n) ++tab_level;
+    uint32_t *tab = info + 8;
+    if (tab_level > 0) {
+        const uint32_t cubes = 1u << (3 * tab_level);
+        hipLaunchKernelGGL(k_cube_starts, dim3(blocks((size_t)cubes + 1, 256)), dim3(256), 0, c.stream, c.keys,
+                           (uint32_t)n, 3 * (kKeyBits - tab_level), cubes, tab);
+        TPM_HIP(hipGetLastError());
+    }
     // A level holds at most min(8^level, n) nodes (each holds a particle); the node-parallel kernels
     // get about one thread per possible node, so the deep levels' latency-bound loops (key loads,
     // record loads) have their iterations in flight at once, and the shallow levels launch little.
@@ -418,13 +451,13 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
     for (int level = 0; level <= c.tree_depth; ++level) {
         const unsigned grid = level_grid(level);
         hipLaunchKernelGGL(k_count_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level,
-                           c.tree_depth, ref_rule, keep_whole, nch);
+                           c.tree_depth, ref_rule, keep_whole, tab, tab_level, nch);
         hipLaunchKernelGGL(k_scan_partials, dim3(kScanBlocks), dim3(256), 0, c.stream, nch, lv, level, part);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlocks), 0, c.stream, part, lv, level, (uint32_t)t.cap,
                            info);
         hipLaunchKernelGGL(k_scan_finish, dim3(kScanBlocks), dim3(256), 0, c.stream, nch, lv, level, part, off);
         hipLaunchKernelGGL(k_make_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level, off,
-                           (uint32_t)t.cap);
+                           (uint32_t)t.cap, tab, tab_level);
         TPM_HIP(hipGetLastError());
     }
     // Centres of mass, deepest level first (levels past the deepest are empty ranges).
