@@ -130,7 +130,7 @@ struct Ctx {
     // Mesh buffers.
     unsigned long long *grid = nullptr;  // fixed-point mass grid, m words
     float2 *specA = nullptr, *specB = nullptr, *specC = nullptr;  // half spectra, hs each
-    float *force = nullptr;    // gx, gy, gz grids (3 m)
+    float *force = nullptr;    // (gx, gy, gz) per mesh point (3 m)
     float *delta = nullptr;    // density contrast grid (m)
 
     // FP64 judge buffers (allocated on first use).
@@ -163,6 +163,11 @@ struct Ctx {
     int fft_path = 0;           // 0: wavefront transforms where the mesh side has them (512), 1: LDS line kernels
     uint32_t *tree_scan = nullptr;  // per-level child counts, offsets, scan partials, level table
     uint32_t *tree_info = nullptr;  // [0] node count, [1] capacity exceeded
+    // First sorted index of every level-`tree_tab_level` cube, from the last octree build over
+    // `tree_tab_n` particles (0: none).
+    const uint32_t *tree_tab = nullptr;
+    int tree_tab_level = 0;
+    size_t tree_tab_n = 0;
     uint32_t *ngroups_dev = nullptr;  // walk group count (device)
     uint32_t *walk_ctr = nullptr;     // walk group counters, one per XCD range (16 words)
     uint32_t *dep_start = nullptr;    // per deposit cube: first sorted index, item count, first item (3 (8^dep_level + 1))

@@ -435,6 +435,9 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
     int tab_level = 0;
     while (tab_level < c.tree_depth && std::pow(8.0, (double)(tab_level + 1)) * 16.0 <= (double)n) ++tab_level;
     uint32_t *tab = info + 8;
+    c.tree_tab = tab;
+    c.tree_tab_level = tab_level;
+    c.tree_tab_n = n;
     if (tab_level > 0) {
         const uint32_t cubes = 1u << (3 * tab_level);
         hipLaunchKernelGGL(k_cube_starts, dim3(blocks((size_t)cubes + 1, 256)), dim3(256), 0, c.stream, c.keys,
@@ -578,10 +581,14 @@ void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
         // Few cubes (about one per 64 or 512 particles): the first particle of every cube by binary
         // search, the groups per cube, an exclusive scan over the cubes and the group starts.
         const uint32_t cubes = (uint32_t)cubes_d;
-        uint32_t *starts = c.gscratch;
+        const uint32_t *starts = c.gscratch;
         uint32_t *gbase = c.gscratch + c.max_n;
-        hipLaunchKernelGGL(k_cube_starts, dim3(blocks((size_t)cubes + 1, 256)), dim3(256), 0, c.stream, keys,
-                           (uint32_t)n, shift, cubes, starts);
+        if (keys == c.keys && n == c.tree_tab_n && level == c.tree_tab_level && c.tree_tab) {
+            starts = c.tree_tab;  // the octree build's table of the same cubes
+        } else {
+            hipLaunchKernelGGL(k_cube_starts, dim3(blocks((size_t)cubes + 1, 256)), dim3(256), 0, c.stream, keys,
+                               (uint32_t)n, shift, cubes, c.gscratch);
+        }
         size_t bytes = c.gscan_bytes;
         const CubeIter ki(rocprim::counting_iterator<uint32_t>(0), CubeGroups{starts, cubes});
         TPM_HIP(rocprim::exclusive_scan(c.gscan_tmp, bytes, ki, gbase, 0u, (size_t)cubes + 1, rocprim::plus<uint32_t>(),
