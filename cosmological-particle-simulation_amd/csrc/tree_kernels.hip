@@ -17,6 +17,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/reverse_iterator.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
 
 #include "tpm_internal.h"
@@ -393,6 +394,31 @@ void sort_particles(Ctx &c, const double *d_pos, size_t n) {
 __global__ void k_cube_starts(const uint64_t *__restrict__ keys, uint32_t n, int shift, uint32_t cubes,
                               uint32_t *__restrict__ starts);
 
+__global__ void k_fill_u32(uint32_t *__restrict__ p, size_t count, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
+// tab[c] = i for the first sorted particle i of every non-empty cube c (one writer per cube).
+__global__ void k_cube_mark(const uint64_t *__restrict__ keys, uint32_t n, int shift, uint32_t *__restrict__ tab) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t c = keys[i] >> shift;
+    if (i == 0 || (keys[i - 1] >> shift) != c) tab[c] = i;
+}
+
+using RevIter = rocprim::reverse_iterator<uint32_t *>;
+
+// Temporary storage of the suffix-minimum scan over up to `count` table entries.
+size_t cube_table_scan_bytes(size_t count) {
+    size_t bytes = 0;
+    uint32_t *p = nullptr;
+    if (rocprim::inclusive_scan(nullptr, bytes, RevIter(p + count), RevIter(p + count), count,
+                                rocprim::minimum<uint32_t>()) != hipSuccess)
+        throw Error{TPM_E_DEVICE, "scan size query failed"};
+    return bytes;
+}
+
 // Tree arrays for up to `cap` nodes (allocated once per capacity; never resized during a build).
 static void ensure_tree(Ctx &c, size_t cap) {
     Tree &t = c.tree;
@@ -429,20 +455,25 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
     TPM_HIP(hipGetLastError());
     const uint32_t ref_rule = (c.p.flags & TPM_FLAG_REF_LEAF_RULE) ? 1u : 0u;
     const uint32_t keep_whole = group_walk_only ? kWalkLeaf : 0u;
-    // The upper levels take their octant boundaries from the first sorted index of every cube of
-    // level T, the deepest level with at most n / 16 cubes (one binary search per cube, all in
-    // parallel), instead of searching per node.
+    // The levels above T take their octant boundaries from a table of the first sorted index of
+    // every level-T cube, instead of searching per node. T is the deepest level with at most n / 8
+    // cubes; the table is built particle-parallel: every cube's first particle marks its entry
+    // (entries preset to n), then a suffix minimum gives empty cubes the next cube's start.
     int tab_level = 0;
-    while (tab_level < c.tree_depth && std::pow(8.0, (double)(tab_level + 1)) * 16.0 <= (double)n) ++tab_level;
+    while (tab_level < c.tree_depth && std::pow(8.0, (double)(tab_level + 1)) * 8.0 <= (double)n) ++tab_level;
     uint32_t *tab = info + 8;
     c.tree_tab = tab;
     c.tree_tab_level = tab_level;
     c.tree_tab_n = n;
     if (tab_level > 0) {
-        const uint32_t cubes = 1u << (3 * tab_level);
-        hipLaunchKernelGGL(k_cube_starts, dim3(blocks((size_t)cubes + 1, 256)), dim3(256), 0, c.stream, c.keys,
-                           (uint32_t)n, 3 * (kKeyBits - tab_level), cubes, tab);
+        const size_t entries = ((size_t)1 << (3 * tab_level)) + 1;
+        hipLaunchKernelGGL(k_fill_u32, dim3(blocks(entries, 256)), dim3(256), 0, c.stream, off, entries, (uint32_t)n);
+        hipLaunchKernelGGL(k_cube_mark, dim3(blocks(n, 256)), dim3(256), 0, c.stream, c.keys, (uint32_t)n,
+                           3 * (kKeyBits - tab_level), off);
         TPM_HIP(hipGetLastError());
+        size_t bytes = c.gscan_bytes;
+        TPM_HIP(rocprim::inclusive_scan(c.gscan_tmp, bytes, RevIter(off + entries), RevIter(tab + entries), entries,
+                                        rocprim::minimum<uint32_t>(), c.stream));
     }
     // A level holds at most min(8^level, n) nodes (each holds a particle); the node-parallel kernels
     // get about one thread per possible node, so the deep levels' latency-bound loops (key loads,
@@ -523,23 +554,26 @@ __global__ void k_cube_starts(const uint64_t *__restrict__ keys, uint32_t n, int
 
 // Groups per cube: ceil(count / kGroup) (entry `cubes` is 0, so its exclusive scan is the total).
 struct CubeGroups {
-    const uint32_t *starts;
+    const uint32_t *starts;  // entry k << sh is cube k's first particle
     uint32_t cubes;
+    int sh;
     __device__ uint32_t operator()(uint32_t k) const {
-        return k < cubes ? (starts[k + 1] - starts[k] + (uint32_t)kGroup - 1) / (uint32_t)kGroup : 0u;
+        return k < cubes ? (starts[(size_t)(k + 1) << sh] - starts[(size_t)k << sh] + (uint32_t)kGroup - 1) /
+                               (uint32_t)kGroup
+                         : 0u;
     }
 };
 
 // Cube k's groups start every kGroup particles from its first; the last thread writes the end.
-__global__ void k_cube_gstart(const uint32_t *__restrict__ starts, const uint32_t *__restrict__ gbase, uint32_t cubes,
-                              uint32_t *__restrict__ gstart, uint32_t *__restrict__ ngroups) {
+__global__ void k_cube_gstart(const uint32_t *__restrict__ starts, int sh, const uint32_t *__restrict__ gbase,
+                              uint32_t cubes, uint32_t *__restrict__ gstart, uint32_t *__restrict__ ngroups) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= cubes) return;
-    const uint32_t s0 = starts[k], s1 = starts[k + 1];
+    const uint32_t s0 = starts[(size_t)k << sh], s1 = starts[(size_t)(k + 1) << sh];
     uint32_t g = gbase[k];
     for (uint32_t p = s0; p < s1; p += (uint32_t)kGroup) gstart[g++] = p;
     if (k == cubes - 1) {
-        gstart[gbase[cubes]] = starts[cubes];
+        gstart[gbase[cubes]] = starts[(size_t)cubes << sh];
         *ngroups = gbase[cubes];
     }
 }
@@ -553,14 +587,14 @@ size_t group_scan_bytes(size_t max_n) {
     uint32_t *p = nullptr;
     const SegIter si(rocprim::counting_iterator<uint32_t>(0), GroupSegStart{nullptr, 0});
     const ChunkIter ci(rocprim::counting_iterator<uint32_t>(0), GroupChunkStart{nullptr});
-    const CubeIter ki(rocprim::counting_iterator<uint32_t>(0), CubeGroups{nullptr, 0});
+    const CubeIter ki(rocprim::counting_iterator<uint32_t>(0), CubeGroups{nullptr, 0, 0});
     size_t f = 0;
     if (rocprim::inclusive_scan(nullptr, a, si, p, max_n, rocprim::maximum<uint32_t>()) != hipSuccess ||
         rocprim::exclusive_scan(nullptr, b, ci, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess ||
         rocprim::exclusive_scan(nullptr, e, p, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess ||
         rocprim::exclusive_scan(nullptr, f, ki, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess)
         throw Error{TPM_E_DEVICE, "scan size query failed"};
-    return std::max(std::max(a, f), std::max(b, e));
+    return std::max(std::max(std::max(a, f), std::max(b, e)), cube_table_scan_bytes(max_n + 1));
 }
 
 void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
@@ -582,18 +616,21 @@ void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
         // search, the groups per cube, an exclusive scan over the cubes and the group starts.
         const uint32_t cubes = (uint32_t)cubes_d;
         const uint32_t *starts = c.gscratch;
+        int sh = 0;
         uint32_t *gbase = c.gscratch + c.max_n;
-        if (keys == c.keys && n == c.tree_tab_n && level == c.tree_tab_level && c.tree_tab) {
-            starts = c.tree_tab;  // the octree build's table of the same cubes
+        if (keys == c.keys && n == c.tree_tab_n && level <= c.tree_tab_level && c.tree_tab) {
+            // The octree build's table of the same or finer cubes: cube k starts at entry k << sh.
+            starts = c.tree_tab;
+            sh = 3 * (c.tree_tab_level - level);
         } else {
             hipLaunchKernelGGL(k_cube_starts, dim3(blocks((size_t)cubes + 1, 256)), dim3(256), 0, c.stream, keys,
                                (uint32_t)n, shift, cubes, c.gscratch);
         }
         size_t bytes = c.gscan_bytes;
-        const CubeIter ki(rocprim::counting_iterator<uint32_t>(0), CubeGroups{starts, cubes});
+        const CubeIter ki(rocprim::counting_iterator<uint32_t>(0), CubeGroups{starts, cubes, sh});
         TPM_HIP(rocprim::exclusive_scan(c.gscan_tmp, bytes, ki, gbase, 0u, (size_t)cubes + 1, rocprim::plus<uint32_t>(),
                                         c.stream));
-        hipLaunchKernelGGL(k_cube_gstart, dim3(blocks(cubes, 256)), dim3(256), 0, c.stream, starts, gbase, cubes,
+        hipLaunchKernelGGL(k_cube_gstart, dim3(blocks(cubes, 256)), dim3(256), 0, c.stream, starts, sh, gbase, cubes,
                            c.gstart, c.ngroups_dev);
         TPM_HIP(hipGetLastError());
         return;
