@@ -269,26 +269,41 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_interp3(const double *__restrict__ pos, const uint32_t *__restrict__ order,
                                                  size_t np, int n, double box, const float *__restrict__ g, size_t m,
                                                  float *__restrict__ acc) {
-    const size_t i = (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    Stencil<T> st;
-    cic_stencil<T>((T)pos[3 * i], (T)pos[3 * i + 1], (T)pos[3 * i + 2], n, (T)box, st);
-    float w[8];
+    // Two particles per thread (256 apart) with their gathers in flight together. g holds
+    // (x, y, z) per mesh point: one record per corner.
+    const size_t base = (size_t)xcd_block(blockIdx.x, gridDim.x) * 512 + threadIdx.x;
+    if (base >= np) return;
+    size_t ii[2] = {base, min(base + 256, np - 1)};
+    Stencil<T> st[2];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) w[c] = (float)st.w[c];
-    const size_t o = order ? order[i] : i;
-    // g holds (x, y, z) per mesh point: one record per corner.
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int u = 0; u < 2; ++u)
+        cic_stencil<T>((T)pos[3 * ii[u]], (T)pos[3 * ii[u] + 1], (T)pos[3 * ii[u] + 2], n, (T)box, st[u]);
+    float v[2][8][3];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const float *gc = g + 3 * (size_t)st.idx[c];
-        sx = sx + gc[0] * w[c];
-        sy = sy + gc[1] * w[c];
-        sz = sz + gc[2] * w[c];
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float *gc = g + 3 * (size_t)st[u].idx[c];
+            v[u][c][0] = gc[0];
+            v[u][c][1] = gc[1];
+            v[u][c][2] = gc[2];
+        }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        if (u == 1 && base + 256 >= np) break;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float w = (float)st[u].w[c];
+            sx = sx + v[u][c][0] * w;
+            sy = sy + v[u][c][1] * w;
+            sz = sz + v[u][c][2] * w;
+        }
+        const size_t o = order ? order[ii[u]] : ii[u];
+        acc[3 * o] = sx;
+        acc[3 * o + 1] = sy;
+        acc[3 * o + 2] = sz;
     }
-    acc[3 * o] = sx;
-    acc[3 * o + 1] = sy;
-    acc[3 * o + 2] = sz;
 }
 
 template <typename T>
@@ -434,10 +449,10 @@ void grid_to_float(Ctx &c, size_t n_particles, float *delta_out, float *mass_out
 void interp_force(Ctx &c, size_t n, bool f32_weights, float *d_acc) {
     if (n == 0) return;
     if (f32_weights)
-        hipLaunchKernelGGL(k_interp3<float>, dim3(blocks(n, 256)), dim3(256), 0, c.stream, c.spos, c.idx, n, c.n,
+        hipLaunchKernelGGL(k_interp3<float>, dim3(blocks(n, 512)), dim3(256), 0, c.stream, c.spos, c.idx, n, c.n,
                            c.p.box_size, c.force, c.m, d_acc);
     else
-        hipLaunchKernelGGL(k_interp3<double>, dim3(blocks(n, 256)), dim3(256), 0, c.stream, c.spos, c.idx, n, c.n,
+        hipLaunchKernelGGL(k_interp3<double>, dim3(blocks(n, 512)), dim3(256), 0, c.stream, c.spos, c.idx, n, c.n,
                            c.p.box_size, c.force, c.m, d_acc);
     TPM_HIP(hipGetLastError());
 }
