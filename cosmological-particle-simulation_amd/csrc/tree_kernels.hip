@@ -67,17 +67,27 @@ __global__ void k_morton_hi(const double *__restrict__ pos, size_t n, double box
     idx[i] = (uint32_t)i;
 }
 
-// Positions in sorted order; with `keys`, also the full keys of the sorted particles.
-__global__ void k_gather(const double *__restrict__ pos, const uint32_t *__restrict__ idx, size_t n, double box,
-                         double *__restrict__ spos, uint64_t *__restrict__ keys) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const size_t j = idx[i];
-    const double p[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
-    spos[3 * i] = p[0];
-    spos[3 * i + 1] = p[1];
-    spos[3 * i + 2] = p[2];
-    if (keys) keys[i] = morton_key(p, box);
+// Positions in sorted order; with `keys`, also the full keys of the sorted particles. Two
+// particles per thread (256 apart), their loads in flight together.
+__global__ void __launch_bounds__(256) k_gather(const double *__restrict__ pos, const uint32_t *__restrict__ idx, size_t n,
+                                                double box, double *__restrict__ spos, uint64_t *__restrict__ keys) {
+    const size_t base = (size_t)blockIdx.x * 512 + threadIdx.x;
+    if (base >= n) return;
+    const bool two = base + 256 < n;
+    const size_t i1 = two ? base + 256 : base;
+    const size_t j0 = idx[base], j1 = idx[i1];
+    const double p0[3] = {pos[3 * j0], pos[3 * j0 + 1], pos[3 * j0 + 2]};
+    const double p1[3] = {pos[3 * j1], pos[3 * j1 + 1], pos[3 * j1 + 2]};
+    spos[3 * base] = p0[0];
+    spos[3 * base + 1] = p0[1];
+    spos[3 * base + 2] = p0[2];
+    if (keys) keys[base] = morton_key(p0, box);
+    if (two) {
+        spos[3 * i1] = p1[0];
+        spos[3 * i1 + 1] = p1[1];
+        spos[3 * i1 + 2] = p1[2];
+        if (keys) keys[i1] = morton_key(p1, box);
+    }
 }
 
 // ---------------------------------------------------------------------------- tree build
@@ -373,7 +383,7 @@ void sort_particles(Ctx &c, const double *d_pos, size_t n) {
         rocprim::double_buffer<uint32_t> vb(c.idx, c.idx_alt);
         TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 30 - 3 * c.tree_depth, 30, c.stream));
         if (vb.current() != c.idx) std::swap(c.idx, c.idx_alt);
-        hipLaunchKernelGGL(k_gather, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.p.box_size,
+        hipLaunchKernelGGL(k_gather, dim3(blocks(n, 512)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.p.box_size,
                            c.spos, c.keys);
         TPM_HIP(hipGetLastError());
         return;
@@ -386,7 +396,7 @@ void sort_particles(Ctx &c, const double *d_pos, size_t n) {
                                       c.stream));
     if (kb.current() != c.keys) std::swap(c.keys, c.keys_alt);
     if (vb.current() != c.idx) std::swap(c.idx, c.idx_alt);
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.p.box_size, c.spos,
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n, 512)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.p.box_size, c.spos,
                        (uint64_t *)nullptr);
     TPM_HIP(hipGetLastError());
 }
