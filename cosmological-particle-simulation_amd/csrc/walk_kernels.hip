@@ -34,7 +34,7 @@
 
 namespace tpm {
 
-constexpr int kSTab = 1024;  // intervals of the S(u) table on [0, u_cut]
+constexpr int kSTab = 1024;  // intervals of the S table on [0, s_max]
 
 struct WalkArgs {
     const double *spos;
@@ -175,13 +175,14 @@ __device__ __forceinline__ double box_dist2(double dx, double dy, double dz, dou
     return ex * ex + ey * ey + ez * ez;
 }
 
-// Wavefronts per workgroup; each walks its own groups. Eight share one copy of the S table, so
-// three workgroups (six wavefronts per SIMD) fit in the CU's LDS.
-constexpr int kWaves = 8;
+// Wavefronts per workgroup; each walks its own groups. Twelve share one copy of the S table, so
+// two workgroups (six wavefronts per SIMD) fit in the CU's LDS with the largest stacks.
+constexpr int kWaves = 12;
 // Pops are limited so the stack stays below kStackCap + 7 * depth, within kStackAlloc for any tree.
-// A smaller cap costs more iterations only when the stack is nearly full; it leaves room in LDS
-// for more resident wavefronts, which the walk's latency-bound traversal needs more.
-constexpr int kStackCap = 320;
+// The cap bounds how many nodes one traversal iteration may pop ((kStackCap - size) / 7), so a
+// larger stack means fewer iterations per group (the traversal's instructions are a fifth of the
+// walk's); 472 is what the LDS holds at two workgroups per CU.
+constexpr int kStackCap = 472;
 constexpr int kStackAlloc = kStackCap + 192;
 constexpr int kRangeCap = 96;
 constexpr int kNodeCap = 96;
