@@ -109,7 +109,7 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
     // stay leaves (more direct sums). Judged on the whole evaluation.
     int best_depth = depth0;
     double best_total = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr);
-    for (int depth : {16, 12, 10}) {
+    for (int depth : {16, 12, 10, 8}) {
         if (depth == depth0) continue;
         c.tree_depth = depth;
         const double tt = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr);
@@ -406,7 +406,7 @@ int tpm_set_tuning(tpm_ctx *ctx, const tpm_tuning *t) {
     if (!ctx || !t) return TPM_E_INVALID;
     Ctx &c = tpm_ctx_internal(ctx);
     if (t->fft_cap < 1 || t->fft_cap > 64 || t->fft_path < 0 || t->fft_path > 1) return TPM_E_INVALID;
-    if (t->tree_depth < 10 || t->tree_depth > tpm::kMaxDepth) return TPM_E_INVALID;
+    if (t->tree_depth < 8 || t->tree_depth > tpm::kMaxDepth) return TPM_E_INVALID;
     if (t->small_node_direct < 0 || t->small_node_direct > 1) return TPM_E_INVALID;
     c.tree_depth = t->tree_depth;
     c.small_direct = t->small_node_direct;

@@ -158,7 +158,7 @@ struct Ctx {
     int group_level = 2;
     int group_level_shift = 0;  // autotuned offset of the group cube level
     int fft_cap = 16;           // autotuned cap on FFT lines per workgroup (LDS line kernels)
-    int tree_depth = kMaxDepth; // octree depth limit (autotuned): deeper nodes stay leaves
+    int tree_depth = kMaxDepth; // octree depth limit (autotuned): deeper nodes stay leaves (see sort_depth)
     int small_direct = 0;       // autotuned: small nodes passing the opening criterion summed directly
     int fft_path = 0;           // 0: wavefront transforms where the mesh side has them (512), 1: LDS line kernels
     uint32_t *tree_scan = nullptr;  // per-level child counts, offsets, scan partials, level table
@@ -179,9 +179,14 @@ struct Ctx {
 // ---------------------------------------------------------------------------------------
 // Launchers (all enqueue on ctx.stream).
 
-// Morton keys of positions (FP64, caller order) and a stable radix sort of (key, index);
-// then spos = positions gathered into key order.
+// Morton keys of positions (FP64, caller order) and a stable radix sort of (key, index) over the
+// key bits of sort_depth(c, n) levels; then spos = positions gathered into key order.
 void sort_particles(Ctx &c, const double *d_pos, size_t n);
+// Morton level of the walk groups' cubes for n particles (the tuned shift applied).
+int walk_group_level(const Ctx &c, size_t n);
+// Levels the sort orders and the octree splits: the tuned depth limit, raised to the finest cube
+// level that is looked up in the sorted keys (the deposit cubes and the walk groups).
+int sort_depth(const Ctx &c, size_t n);
 
 // Fixed-point CIC deposit of positions into c.grid. f32_weights: compute the stencil in FP32
 // from the positions rounded to float (the reference's arithmetic). sorted: positions are in

@@ -367,12 +367,25 @@ __global__ void k_tree_root(TreeView t, uint32_t n, float half, uint32_t *__rest
 
 static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+int walk_group_level(const Ctx &c, size_t n) {
+    // Group cubes hold about 512 particles at the mean density (8 particle spacings on a side
+    // when mesh = particle grid), and are never larger than L/4.
+    int level = 2;
+    while (level < kMaxDepth && ((double)n / std::pow(8.0, level + 1)) >= 512.0) ++level;
+    return std::max(2, std::min(kMaxDepth, level + c.group_level_shift));
+}
+
+int sort_depth(const Ctx &c, size_t n) {
+    return std::min(kMaxDepth, std::max(c.tree_depth, std::max(c.dep_level, walk_group_level(c, n))));
+}
+
 void sort_particles(Ctx &c, const double *d_pos, size_t n) {
     if (n == 0) return;
     size_t bytes = c.sort_tmp_bytes;
     // Only the key bits of the octree's levels are sorted (a shallower tree sorts fewer bits; the
     // sort is stable, so the order below that depth is the input order).
-    if (c.tree_depth <= 10) {
+    const int depth = sort_depth(c, n);
+    if (depth <= 10) {
         // At most 30 key bits: 32-bit keys halve the key traffic of the radix passes; the gather
         // recomputes the full keys of the sorted particles.
         uint32_t *k32 = reinterpret_cast<uint32_t *>(c.keys), *k32_alt = reinterpret_cast<uint32_t *>(c.keys_alt);
@@ -381,7 +394,7 @@ void sort_particles(Ctx &c, const double *d_pos, size_t n) {
         TPM_HIP(hipGetLastError());
         rocprim::double_buffer<uint32_t> kb(k32, k32_alt);
         rocprim::double_buffer<uint32_t> vb(c.idx, c.idx_alt);
-        TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 30 - 3 * c.tree_depth, 30, c.stream));
+        TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 30 - 3 * depth, 30, c.stream));
         if (vb.current() != c.idx) std::swap(c.idx, c.idx_alt);
         hipLaunchKernelGGL(k_gather, dim3(blocks(n, 512)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.p.box_size,
                            c.spos, c.keys);
@@ -392,7 +405,7 @@ void sort_particles(Ctx &c, const double *d_pos, size_t n) {
     TPM_HIP(hipGetLastError());
     rocprim::double_buffer<uint64_t> kb(c.keys, c.keys_alt);
     rocprim::double_buffer<uint32_t> vb(c.idx, c.idx_alt);
-    TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 3 * (kKeyBits - c.tree_depth), 3 * kKeyBits,
+    TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 3 * (kKeyBits - depth), 3 * kKeyBits,
                                       c.stream));
     if (kb.current() != c.keys) std::swap(c.keys, c.keys_alt);
     if (vb.current() != c.idx) std::swap(c.idx, c.idx_alt);
@@ -452,6 +465,7 @@ size_t tree_capacity(size_t n) { return std::max<size_t>(1024, 2 * n + 1024); }
 
 void build_tree(Ctx &c, size_t n, bool group_walk_only) {
     Tree &t = c.tree;
+    const int depth = sort_depth(c, n);  // the levels the sort ordered
     ensure_tree(c, tree_capacity(n));
     uint32_t *nch = c.tree_scan;
     uint32_t *off = nch + t.cap;
@@ -470,7 +484,7 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
     // cubes; the table is built particle-parallel: every cube's first particle marks its entry
     // (entries preset to n), then a suffix minimum gives empty cubes the next cube's start.
     int tab_level = 0;
-    while (tab_level < c.tree_depth && std::pow(8.0, (double)(tab_level + 1)) * 8.0 <= (double)n) ++tab_level;
+    while (tab_level < depth && std::pow(8.0, (double)(tab_level + 1)) * 8.0 <= (double)n) ++tab_level;
     uint32_t *tab = info + 8;
     c.tree_tab = tab;
     c.tree_tab_level = tab_level;
@@ -492,10 +506,10 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
         const double bound = std::min(std::pow(8.0, (double)level), (double)n);
         return (unsigned)std::max(1.0, std::min(std::ceil(bound / 256.0), 65536.0));
     };
-    for (int level = 0; level <= c.tree_depth; ++level) {
+    for (int level = 0; level <= depth; ++level) {
         const unsigned grid = level_grid(level);
         hipLaunchKernelGGL(k_count_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level,
-                           c.tree_depth, ref_rule, keep_whole, tab, tab_level, nch);
+                           depth, ref_rule, keep_whole, tab, tab_level, nch);
         hipLaunchKernelGGL(k_scan_partials, dim3(kScanBlocks), dim3(256), 0, c.stream, nch, lv, level, part);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlocks), 0, c.stream, part, lv, level, (uint32_t)t.cap,
                            info);
@@ -505,7 +519,7 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
         TPM_HIP(hipGetLastError());
     }
     // Centres of mass, deepest level first (levels past the deepest are empty ranges).
-    for (int level = c.tree_depth + 1; level >= 0; --level) {
+    for (int level = depth + 1; level >= 0; --level) {
         hipLaunchKernelGGL(k_com_level, dim3(level_grid(level)), dim3(256), 0, c.stream, c.spos, t.view(), lv, level);
         TPM_HIP(hipGetLastError());
     }
@@ -613,11 +627,7 @@ void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
         zero_fill(c, c.ngroups_dev, sizeof(uint32_t));
         return;
     }
-    // Group cubes hold about 512 particles at the mean density (8 particle spacings on a side
-    // when mesh = particle grid), and are never larger than L/4.
-    int level = 2;
-    while (level < kMaxDepth && ((double)n / std::pow(8.0, level + 1)) >= 512.0) ++level;
-    level = std::max(2, std::min(kMaxDepth, level + c.group_level_shift));
+    const int level = walk_group_level(c, n);
     c.group_level = level;
     const int shift = 3 * (kKeyBits - level);
     const double cubes_d = std::pow(8.0, (double)level);

@@ -353,7 +353,8 @@ typedef struct tpm_tuning {
     int32_t fft_cap;
     int32_t group_level_shift;
     int32_t fft_path; /* 0: wavefront transforms where available (mesh side 512), 1: LDS line kernels */
-    int32_t tree_depth; /* octree depth limit, 10..21 (default 21): deeper nodes stay leaves and fewer key bits are sorted */
+    int32_t tree_depth; /* octree depth limit, 8..21 (default 21): deeper nodes stay leaves and fewer key bits are
+                           sorted; raised where the deposit cubes or the walk groups need finer sorted levels */
     int32_t small_node_direct; /* 0 (default): nodes of at most 16 particles that pass the opening criterion are
                                   monopoles; 1: they are summed directly too (more pairs, no monopole) */
     double ms_default; /* force evaluation with the default knobs */

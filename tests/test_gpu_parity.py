@@ -304,6 +304,32 @@ def test_group_building_paths_agree(treepm):
     assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
 
 
+def test_depth_limit_is_raised_to_the_group_cubes(treepm):
+    # A depth limit of 8 with the group level at 9 (level 2 for 4096 particles, shifted by 7): the
+    # sort is taken to level 9, so each group cube's particles stay contiguous. 4096 particles in
+    # one level-8 cube: sorted only to level 8 they would stay in their random input order and cut
+    # the eight level-9 cubes into thousands of runs; sorted to level 9 each cube of c particles
+    # gives ceil(c / 64) groups, as with no depth limit. At theta = 0 every pair is summed, so the
+    # forces agree to FP32 summation order.
+    rng = np.random.default_rng(5)
+    pos = (25 + rng.random((4096, 3))) * (100.0 / 256)
+    cells = np.floor(pos / (100.0 / 512)).astype(np.int64)
+    _, counts = np.unique(cells, axis=0, return_counts=True)
+    assert len(counts) == 8
+    groups = int(np.sum((counts + 63) // 64))
+    split = treepm.force_split(100.0, 16, 16, 1 / 30, 0.0)
+    out = []
+    for depth in (8, 21):
+        s = solver(treepm, 16, 100.0, split, max_particles=4096)
+        s.set_tuning(16, 7, fft_path=0, tree_depth=depth, small_node_direct=0)
+        s.set_timing(True)
+        a, _ = s.accelerations(pos, 0.02)
+        out.append((a.astype(np.float64), s.stats()["groups"]))
+        s.close()
+    assert out[0][1] == out[1][1] == groups
+    assert max_diff(out[0][0], out[1][0]) / max_norm(out[1][0]) < 1e-5
+
+
 def test_walk_without_counting_gives_identical_forces(treepm):
     # The interaction counter is instrumentation only: switching it off changes no bit of the
     # forces; the interaction total then reads 0 while the other walk statistics stay.
