@@ -68,10 +68,12 @@ __global__ void k_morton_hi(const double *__restrict__ pos, size_t n, double box
 }
 
 // Positions in sorted order; with `keys`, also the full keys of the sorted particles. Two
-// particles per thread (256 apart), their loads in flight together.
+// particles per thread (256 apart), their loads in flight together. Each XCD takes a contiguous
+// run of the sorted order (a compact region of the box), so the lines of input positions that
+// neighbouring cubes share are found in its L2.
 __global__ void __launch_bounds__(256) k_gather(const double *__restrict__ pos, const uint32_t *__restrict__ idx, size_t n,
                                                 double box, double *__restrict__ spos, uint64_t *__restrict__ keys) {
-    const size_t base = (size_t)blockIdx.x * 512 + threadIdx.x;
+    const size_t base = (size_t)xcd_block(blockIdx.x, gridDim.x) * 512 + threadIdx.x;
     if (base >= n) return;
     const bool two = base + 256 < n;
     const size_t i1 = two ? base + 256 : base;
