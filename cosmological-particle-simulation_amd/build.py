@@ -24,8 +24,10 @@ COMMON = [
     "-I", os.path.join(REPO, "include"), "-I", CSRC,
 ]
 # The CIC stencil and interpolation follow the reference's FP32 operation order exactly, so
-# no multiply-add may be fused there.
-PER_FILE = {"mesh_kernels.hip": ["-ffp-contract=off"]}
+# no multiply-add may be fused there. The walk is bound by vector instruction issue; the scheduler
+# that favours instruction-level parallelism over occupancy (the occupancy is fixed by the
+# kernel's own bounds) lets the SIMDs pair more of its instructions: 0.4% faster at 512^3.
+PER_FILE = {"mesh_kernels.hip": ["-ffp-contract=off"], "walk_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 
 
 def compile_one(src):
