@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <mutex>
 
 #include "tpm_internal.h"
 
@@ -520,4 +521,71 @@ void check_finite(Ctx &c, const float *d_acc, size_t count) {
     TPM_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------- deposit without allocation
+
+// The fixed-point mass grid of tpm_deposit_dev_f64_noalloc: a file-scope device array of 16 MiB
+// (meshes of up to 128 cells a side) instead of memory the library allocates. Each GPU holds its
+// own copy from the first use until the process ends. On the host the array's name is not a
+// device address: each device's address comes from hipGetSymbolAddress, looked up once per
+// device (with that device current) on the first call there, and only that address reaches
+// kernels. Every call on one GPU shares the array, so calls that overlap on one GPU race.
+constexpr int kNoAllocMaxMesh = 128;
+static __device__ __align__(256) unsigned long long scratch[(size_t)kNoAllocMaxMesh * kNoAllocMaxMesh *
+                                                             kNoAllocMaxMesh];
+
+// Address of `scratch` on the current device (null and a message on failure).
+static unsigned long long *scratch_on_current_device(const char **why) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static unsigned long long *addr[kMaxDevices] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        *why = "no current HIP device";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            *why = "hipGetSymbolAddress of the deposit scratch grid failed";
+            return nullptr;
+        }
+        addr[dev] = static_cast<unsigned long long *>(p);
+    }
+    return addr[dev];
+}
+
+static int noalloc_fail(int code, const char *msg) {
+    set_global_error(msg);
+    return code;
+}
+
 }  // namespace tpm
+
+extern "C" int tpm_deposit_dev_f64_noalloc(const double *d_pos, size_t n, int32_t mesh_n, double box_size,
+                                           double mass, float *d_cells, void *stream) {
+    using namespace tpm;
+    if (mesh_n < 2 || mesh_n > kNoAllocMaxMesh)
+        return noalloc_fail(TPM_E_INVALID, "mesh_n must be in [2, 128] for the allocation-free deposit");
+    if (!(box_size > 0.0) || !std::isfinite(box_size)) return noalloc_fail(TPM_E_INVALID, "box_size must be positive");
+    if (!std::isfinite(mass)) return noalloc_fail(TPM_E_INVALID, "mass must be finite");
+    if (!d_cells || (n > 0 && !d_pos)) return noalloc_fail(TPM_E_INVALID, "null device pointer");
+    if (n > (size_t)UINT32_MAX) return noalloc_fail(TPM_E_INVALID, "at most 2^32 - 1 particles");
+    const char *why = "";
+    unsigned long long *grid = scratch_on_current_device(&why);
+    if (!grid) return noalloc_fail(TPM_E_DEVICE, why);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t m = (size_t)mesh_n * mesh_n * mesh_n;
+    // The grid keeps the previous call's sums: clear the cells this call uses (a fill kernel, not
+    // a memset, so the call can be captured into a HIP graph).
+    const size_t words = 2 * m;
+    hipLaunchKernelGGL(k_zero32, dim3((unsigned)std::min<size_t>(blocks(words, 256), 8192)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t *>(grid), words);
+    if (n > 0)
+        hipLaunchKernelGGL(k_deposit<double>, dim3(blocks(n, 256)), dim3(256), 0, s, d_pos, n, (int)mesh_n, box_size,
+                           mass, grid);
+    hipLaunchKernelGGL(k_grid_to_float, dim3(blocks(m, 256)), dim3(256), 0, s, grid, m, 1.0f, (float *)nullptr,
+                       d_cells);
+    if (hipGetLastError() != hipSuccess) return noalloc_fail(TPM_E_DEVICE, "kernel launch failed");
+    return TPM_OK;
+}

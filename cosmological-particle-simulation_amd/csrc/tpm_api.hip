@@ -2,6 +2,7 @@
 // that replaces Simulation::accelerations (src/integrator.rs:140-161), and the component
 // entry points that mirror the reference's public functions.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
 
@@ -12,7 +13,13 @@
 namespace tpm {
 size_t sort_temp_bytes(size_t max_n);
 
-static std::string g_create_error;
+// Message of the last failure of a call made without a context (tpm_create, the allocation-free
+// deposit); a fixed buffer, so recording it allocates nothing.
+static char g_create_error[256] = "";
+
+void set_global_error(const char *msg) {
+    std::snprintf(g_create_error, sizeof g_create_error, "%s", msg ? msg : "");
+}
 
 __global__ void k_inverse_perm(const uint32_t *__restrict__ idx, size_t n, uint32_t *__restrict__ inv) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -296,19 +303,19 @@ const char *tpm_version(void) { return "tpm 0.1 (gfx950)"; }
 
 int tpm_create(const tpm_params *params, size_t max_particles, tpm_ctx **out) {
     if (!params || !out) {
-        tpm::g_create_error = "null argument";
+        tpm::set_global_error("null argument");
         return TPM_E_INVALID;
     }
     *out = nullptr;
     tpm_ctx *ctx = new (std::nothrow) tpm_ctx();
     if (!ctx) {
-        tpm::g_create_error = "host allocation failed";
+        tpm::set_global_error("host allocation failed");
         return TPM_E_DEVICE;
     }
     try {
         tpm::create(ctx->c, *params, max_particles);
     } catch (const tpm::Error &e) {
-        tpm::g_create_error = e.msg;
+        tpm::set_global_error(e.msg.c_str());
         tpm::free_all(ctx->c);
         delete ctx;
         return e.code;
@@ -325,7 +332,7 @@ void tpm_destroy(tpm_ctx *ctx) {
 }
 
 const char *tpm_last_error(const tpm_ctx *ctx) {
-    if (!ctx) return tpm::g_create_error.c_str();
+    if (!ctx) return tpm::g_create_error;
     return ctx->c.err.c_str();
 }
 

@@ -247,6 +247,8 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
 void finish_checks(Ctx &c);
 
 // Misc.
+// Records the message tpm_last_error(NULL) returns (fixed buffer, no allocation).
+void set_global_error(const char *msg);
 // Zero `bytes` (a multiple of 4) at ptr with a fill kernel (graph-capture safe, unlike memset).
 void zero_fill(Ctx &c, void *ptr, size_t bytes);
 size_t tree_capacity(size_t n);

@@ -49,7 +49,7 @@ EXPORTS = [
     "tpm_growth", "tpm_power_spectrum", "tpm_power_sigma", "tpm_zeldovich_bbks", "tpm_zeldovich_bbks_dev",
     "tpm_snapshot_dev", "tpm_run_settings_default", "tpm_run_settings_sanitized", "tpm_headless_run", "tpm_downsample_grid_dev", "tpm_autotune", "tpm_set_tuning", "tpm_ensemble_create", "tpm_ensemble_destroy", "tpm_ensemble_last_error",
     "tpm_ensemble_ics", "tpm_ensemble_autotune", "tpm_ensemble_run", "tpm_ensemble_get",
-    "tpm_snapshot_write", "tpm_snapshot_read",
+    "tpm_snapshot_write", "tpm_snapshot_read", "tpm_deposit_dev_f64_noalloc",
 ]
 
 
@@ -264,6 +264,7 @@ def load():
         "tpm_ensemble_run": (ctypes.c_int, [P, ctypes.POINTER(Cosmology), D, D, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_double)]),
         "tpm_ensemble_get": (ctypes.c_int, [P, ctypes.c_int, P, P, P]),
+        "tpm_deposit_dev_f64_noalloc": (ctypes.c_int, [P, S, ctypes.c_int32, D, D, P, P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)

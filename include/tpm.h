@@ -122,6 +122,20 @@ int tpm_set_counting(tpm_ctx *ctx, int enabled);
 int tpm_deposit_f32(tpm_ctx *ctx, const float *pos, size_t n, float mass, float *cells);
 int tpm_deposit_f64(tpm_ctx *ctx, const double *pos, size_t n, double mass, float *cells);
 
+/*
+ * Allocation-free variant of tpm_deposit_f64 (no counterpart in the reference): for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc). No context: d_pos (3n FP64 positions) and d_cells (mesh_n^3 floats) are device
+ * pointers; the work is enqueued on `stream` (hipStream_t, NULL = the null stream) without
+ * synchronising. The same fixed-point CIC deposit, so d_cells equals tpm_deposit_f64's output bit
+ * for bit. Its 2^-32 fixed-point grid is a 16 MiB array built into the library (one per GPU, held
+ * until the process ends), so mesh_n is limited to [2, 128]; larger meshes fail with
+ * TPM_E_INVALID before any device work. Calls that overlap on one GPU share that grid and race.
+ * Errors: tpm_last_error(NULL). The first call on a device looks up the array's address there.
+ */
+int tpm_deposit_dev_f64_noalloc(const double *d_pos, size_t n, int32_t mesh_n, double box_size, double mass,
+                                float *d_cells, void *stream);
+
 /* Simulation::density_contrast (src/integrator.rs:117-126). */
 int tpm_density_contrast_f32(tpm_ctx *ctx, const float *pos, size_t n, float *delta);
 
