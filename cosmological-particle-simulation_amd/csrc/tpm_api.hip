@@ -46,6 +46,24 @@ __global__ void k_add3(double *__restrict__ a, const double *__restrict__ b, siz
 
 static inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// One thread: ORs the failure flags of the evaluation just enqueued into the sticky status word.
+__global__ void k_fold_status(const unsigned long long *__restrict__ walk_overflow, const uint32_t *__restrict__ tree_full,
+                              const int *__restrict__ nonfinite, uint32_t *__restrict__ status) {
+    uint32_t s = 0;
+    if (walk_overflow && *walk_overflow) s |= TPM_STATUS_WALK_OVERFLOW;
+    if (tree_full && *tree_full) s |= TPM_STATUS_TREE_CAPACITY;
+    if (nonfinite && *nonfinite) s |= TPM_STATUS_NONFINITE;
+    if (s) atomicOr(status, s);
+}
+
+void fold_status(Ctx &c, bool tree_built) {
+    const uint32_t *tree_full = tree_built && c.tree_info ? c.tree_info + 1 : nullptr;
+    const int *nonfinite = (c.p.flags & TPM_FLAG_CHECK_FINITE) ? c.flag : nullptr;
+    hipLaunchKernelGGL(k_fold_status, dim3(1), dim3(1), 0, c.stream, c.counter + 1, tree_full, nonfinite, c.status);
+    TPM_HIP(hipGetLastError());
+    c.status_stream = c.stream;
+}
+
 static void validate(const tpm_params &p) {
     TPM_REQUIRE(p.mesh_n >= 2 && p.mesh_n <= 1024, "mesh_n must be in [2, 1024]");
     TPM_REQUIRE(std::isfinite(p.box_size) && p.box_size > 0, "box_size must be positive");
@@ -63,7 +81,7 @@ static void free_all(Ctx &c) {
     for (hipEvent_t &e : c.walk_ev)
         if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.sort_tmp, c.spos,
-                    c.counter, c.flag, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
+                    c.counter, c.flag, c.status, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
                     c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.tree_scan,
                     c.ngroups_dev, c.walk_ctr, c.dep_start, c.dep_scan_tmp, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp, c.stab,
                     c.acc_sr, c.members, c.reject, c.mlp.params, c.mlp.grads, c.mlp.m1, c.mlp.m2};
@@ -117,6 +135,8 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     }
     TPM_HIP(hipMalloc(&c.counter, 8 * sizeof(unsigned long long)));
     TPM_HIP(hipMalloc(&c.flag, sizeof(int) * 4));
+    TPM_HIP(hipMalloc(&c.status, sizeof(uint32_t)));
+    TPM_HIP(hipMemset(c.status, 0, sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.grid, c.m * sizeof(unsigned long long)));
     TPM_HIP(hipMalloc(&c.specA, c.hs * sizeof(float2)));
     TPM_HIP(hipMalloc(&c.specB, c.hs * sizeof(float2)));
@@ -207,6 +227,7 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
         tm.mark();
     }
     if (c.p.flags & TPM_FLAG_CHECK_FINITE) check_finite(c, d_acc, 3 * n);
+    fold_status(c, treepm && n > 0);
     if (c.timing) {
         TPM_HIP(hipStreamSynchronize(c.stream));
         c.stats.ms_sort = tm.ms(0, 1);
@@ -499,6 +520,14 @@ int tpm_judge_f64(tpm_ctx *ctx, const double *pos, size_t n, const uint32_t *idx
         TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
         TPM_REQUIRE(theta >= 0 && std::isfinite(theta), "theta must be >= 0");
         tpm::upload_positions(c, pos, n, false);
+        // The judge's octree is the full-depth one whatever the tuned depth limit (the yardstick
+        // must not depend on tuning); the product's next evaluation sorts and builds again.
+        struct DepthScope {
+            Ctx &c;
+            int saved;
+            ~DepthScope() { c.tree_depth = saved; }
+        } depth_scope{c, c.tree_depth};
+        c.tree_depth = tpm::kMaxDepth;
         tpm::sort_particles(c, c.pos, n);
         tpm::deposit_fixed(c, n, 1.0, false, c.spos, true);
         const bool treepm = (c.p.flags & TPM_FLAG_TREEPM) != 0;
@@ -532,6 +561,19 @@ int tpm_judge_f64(tpm_ctx *ctx, const double *pos, size_t n, const uint32_t *idx
         TPM_HIP(hipMemcpyAsync(acc, d_pm, 3 * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         TPM_HIP(hipStreamSynchronize(c.stream));
         TPM_REQUIRE(!bad, "judge target index out of range");
+        return TPM_OK;
+    });
+}
+
+int tpm_get_device_status(tpm_ctx *ctx, uint32_t *status, int clear) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(status != nullptr, "status pointer is null");
+        if (c.status_stream && c.status_stream != c.stream) TPM_HIP(hipStreamSynchronize(c.status_stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        uint32_t s = 0;
+        TPM_HIP(hipMemcpy(&s, c.status, sizeof(s), hipMemcpyDeviceToHost));
+        if (clear) TPM_HIP(hipMemset(c.status, 0, sizeof(uint32_t)));
+        *status = s;
         return TPM_OK;
     });
 }

@@ -126,6 +126,8 @@ struct Ctx {
     double *spos = nullptr;    // 3 max_n positions in Morton order
     unsigned long long *counter = nullptr;  // interaction counter + scratch words
     int *flag = nullptr;       // non-finite flag
+    uint32_t *status = nullptr;  // sticky TPM_STATUS_* bits of every evaluation since the last clear
+    hipStream_t status_stream = nullptr;  // stream of the last evaluation that folded into `status`
 
     // Mesh buffers.
     unsigned long long *grid = nullptr;  // fixed-point mass grid, m words
@@ -242,6 +244,9 @@ void downsample_grid(Ctx &c, const float *delta, float *out);
 // Force evaluation on device-resident FP64 positions (tpm_api.hip); no host synchronisation
 // unless timing is on.
 void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float *d_acc, float *d_delta);
+// Enqueues the fold of this evaluation's failure flags (walk stack overflow, octree capacity when
+// the tree was built, non-finite accelerations when checked) into the sticky c.status word.
+void fold_status(Ctx &c, bool tree_built);
 // Synchronises c.stream and throws on a walk stack overflow, octree capacity overflow or (with
 // TPM_FLAG_CHECK_FINITE) a non-finite acceleration of the last evaluation.
 void finish_checks(Ctx &c);

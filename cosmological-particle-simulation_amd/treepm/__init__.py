@@ -35,6 +35,10 @@ FLAG_REF_LEAF_RULE = 2
 FLAG_CHECK_FINITE = 4
 FLAG_REF_WALK = 8
 
+STATUS_WALK_OVERFLOW = 1
+STATUS_TREE_CAPACITY = 2
+STATUS_NONFINITE = 4
+
 # Symbols the header declares (checked by the CPU test suite against the built library).
 EXPORTS = [
     "tpm_version", "tpm_create", "tpm_destroy", "tpm_last_error", "tpm_set_params",
@@ -49,7 +53,7 @@ EXPORTS = [
     "tpm_growth", "tpm_power_spectrum", "tpm_power_sigma", "tpm_zeldovich_bbks", "tpm_zeldovich_bbks_dev",
     "tpm_snapshot_dev", "tpm_run_settings_default", "tpm_run_settings_sanitized", "tpm_headless_run", "tpm_downsample_grid_dev", "tpm_autotune", "tpm_set_tuning", "tpm_ensemble_create", "tpm_ensemble_destroy", "tpm_ensemble_last_error",
     "tpm_ensemble_ics", "tpm_ensemble_autotune", "tpm_ensemble_run", "tpm_ensemble_get",
-    "tpm_snapshot_write", "tpm_snapshot_read", "tpm_deposit_dev_f64_noalloc",
+    "tpm_snapshot_write", "tpm_snapshot_read", "tpm_deposit_dev_f64_noalloc", "tpm_get_device_status",
 ]
 
 
@@ -265,6 +269,7 @@ def load():
                                             ctypes.POINTER(ctypes.c_double)]),
         "tpm_ensemble_get": (ctypes.c_int, [P, ctypes.c_int, P, P, P]),
         "tpm_deposit_dev_f64_noalloc": (ctypes.c_int, [P, S, ctypes.c_int32, D, D, P, P]),
+        "tpm_get_device_status": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -523,6 +528,12 @@ class Solver:
 
     def sync(self):
         self._check(self.lib.tpm_sync(self.h))
+
+    def device_status(self, clear=False):
+        """Sticky device-side failure bits (STATUS_*) of the evaluations since the last clear."""
+        v = ctypes.c_uint32()
+        self._check(self.lib.tpm_get_device_status(self.h, ctypes.byref(v), 1 if clear else 0))
+        return int(v.value)
 
     def snapshot_dev(self, d_pos, d_mom, n, a, e, d_delta, d_pos32=0, d_speed=0, d_density=0, d_rsd=0, stream=0):
         """snapshot (src/runner.rs:262-284) into device buffers (0 = not wanted)."""

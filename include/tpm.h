@@ -109,6 +109,19 @@ int tpm_accelerations_f64(tpm_ctx *ctx, const double *pos, size_t n, double a, f
 int tpm_accelerations_dev_f64(tpm_ctx *ctx, const double *d_pos, size_t n, double a, float *d_acc,
                               float *d_delta, void *stream);
 
+/*
+ * Device-side failures of the evaluations enqueued on ctx since the status was last cleared
+ * (no reference counterpart; the reference cannot fail this way). The *_dev entry points never
+ * wait for the device, so they cannot return these; each evaluation ORs its flags into a sticky
+ * word that this call reads after waiting for the context's stream and the stream of the last
+ * evaluation. clear != 0 resets the word. The host-pointer entry points return such failures as
+ * error codes as well.
+ */
+#define TPM_STATUS_WALK_OVERFLOW 1u  /* a walk's traversal stack overflowed: forces incomplete */
+#define TPM_STATUS_TREE_CAPACITY 2u  /* the octree outgrew its node capacity: forces incomplete */
+#define TPM_STATUS_NONFINITE 4u      /* a non-finite acceleration (only with TPM_FLAG_CHECK_FINITE) */
+int tpm_get_device_status(tpm_ctx *ctx, uint32_t *status, int clear);
+
 /* Statistics of the last force evaluation (device timings need TPM_FLAG timing enabled via
  * tpm_set_timing). */
 int tpm_get_stats(const tpm_ctx *ctx, tpm_stats *out);

@@ -183,7 +183,7 @@ def test_tree_theta_half_close_to_brute_force(treepm, oracle):
         s = solver(treepm, 8, 10.0, split, flags=flags)
         ref_gpu = s.short_range(z["pos3000"], 1.0)
         ora, _ = oracle.tree_accel(z["pos3000"], 10.0, 0.5, 2.5, 0.05, 0.5, 1.0)
-        assert max_diff(fast, z["acc3000"]) / max_norm(z["acc3000"]) < 0.02
+        assert max_diff(ref_gpu, z["acc3000"]) / max_norm(z["acc3000"]) < 0.02
         if flags & treepm.FLAG_REF_LEAF_RULE:
             assert max_diff(ref_gpu, ora) / max_norm(ora) < 1e-4
 
@@ -362,6 +362,21 @@ def test_judge_matches_fp64_oracle(treepm, oracle):
     j = s.judge(pos, idx, a, theta=0.2)
     ref, _, _ = oracle.accelerations(pos, n, 100.0, OMEGA_M, a, dict(split, theta=0.2), dtype=np.float64)
     assert np.max(rel_errors(j, ref[idx])) < 1e-8
+
+
+def test_judge_does_not_depend_on_tuning(treepm):
+    # The judge builds the full-depth octree whatever the tuned depth limit, so its output is the
+    # same before and after tuning the product's octree to depth 8.
+    n = 32
+    pos = _zeldovich_config(treepm, n)
+    split = treepm.force_split(100.0, n, n, 1 / 30, 0.5)
+    idx = np.arange(0, n ** 3, 5, dtype=np.uint32)
+    s = solver(treepm, n, 100.0, split, max_particles=n ** 3)
+    j1 = s.judge(pos, idx, 0.02, theta=0.2)
+    s.set_tuning(16, 0, fft_path=0, tree_depth=8, small_node_direct=0)
+    s.accelerations(pos, 0.02)
+    j2 = s.judge(pos, idx, 0.02, theta=0.2)
+    assert np.array_equal(j1, j2)
 
 
 def test_edge_cases(treepm):
