@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "libtpm.so")
 BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["tpm_api.hip", "mesh_kernels.hip", "fft_kernels.hip", "tree_kernels.hip", "walk_kernels.hip", "ic_kernels.hip", "kdk_kernels.hip", "mlp_kernels.hip", "ensemble.hip", "runner.hip", "snapshot_io.cpp"]
+SOURCES = ["tpm_api.hip", "mesh_kernels.hip", "fft_kernels.hip", "tree_kernels.hip", "sort_kernels.hip", "walk_kernels.hip", "ic_kernels.hip", "kdk_kernels.hip", "mlp_kernels.hip", "ensemble.hip", "runner.hip", "snapshot_io.cpp"]
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
     # Packed FP32 instructions stay off: the compiler would otherwise form v_pk_* from float code.

@@ -418,8 +418,7 @@ void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double
                            nodes, c.dep_start);
         hipLaunchKernelGGL(k_node_items, dim3(blocks(nodes + 1, 256)), dim3(256), 0, c.stream, c.dep_start, nodes,
                            items);
-        size_t bytes = c.dep_scan_bytes;
-        TPM_HIP(rocprim_exclusive_scan_u32(c.dep_scan_tmp, bytes, items, item_start, nodes + 1, c.stream));
+        exclusive_sum_u32(c, items, item_start, nodes + 1);
         // One workgroup per cube for its first item, plus at most n / kDepItem extra items.
         const size_t grid_items = (size_t)nodes + n / kDepItem + 1;
         if (f32_weights)

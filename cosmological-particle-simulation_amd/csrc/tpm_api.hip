@@ -6,12 +6,12 @@
 #include <cstring>
 #include <new>
 
+#include "device_scan.h"
 #include "tpm_internal.h"
 
 #include <algorithm>
 
 namespace tpm {
-size_t sort_temp_bytes(size_t max_n);
 
 // Message of the last failure of a call made without a context (tpm_create, the allocation-free
 // deposit); a fixed buffer, so recording it allocates nothing.
@@ -80,10 +80,10 @@ static void validate(const tpm_params &p) {
 static void free_all(Ctx &c) {
     for (hipEvent_t &e : c.walk_ev)
         if (e) (void)hipEventDestroy(e);
-    void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.sort_tmp, c.spos,
+    void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.radix_table, c.scan_part, c.spos,
                     c.counter, c.flag, c.status, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
                     c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.tree_scan,
-                    c.ngroups_dev, c.walk_ctr, c.dep_start, c.dep_scan_tmp, c.inv, c.judge_buf, c.gstart, c.gscratch, c.gscan_tmp, c.stab,
+                    c.ngroups_dev, c.walk_ctr, c.dep_start, c.inv, c.judge_buf, c.gstart, c.gscratch, c.stab,
                     c.acc_sr, c.members, c.reject, c.mlp.params, c.mlp.grads, c.mlp.m1, c.mlp.m2};
     for (int l = 0; l < 4; ++l) {
         if (c.mlp.wt16[l]) (void)hipFree(c.mlp.wt16[l]);
@@ -116,13 +116,12 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     TPM_HIP(hipMalloc(&c.keys_alt, mn * sizeof(uint64_t)));
     TPM_HIP(hipMalloc(&c.idx, mn * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.idx_alt, mn * sizeof(uint32_t)));
-    c.sort_tmp_bytes = sort_temp_bytes(mn);
-    TPM_HIP(hipMalloc(&c.sort_tmp, c.sort_tmp_bytes ? c.sort_tmp_bytes : 16));
+    c.radix_table_words = radix_tiles(mn) * 256;
+    TPM_HIP(hipMalloc(&c.radix_table, c.radix_table_words * sizeof(uint32_t)));
+    TPM_HIP(hipMalloc(&c.scan_part, kScanParts * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.spos, 3 * mn * sizeof(double)));
     TPM_HIP(hipMalloc(&c.gstart, (mn + 1) * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.gscratch, 2 * mn * sizeof(uint32_t)));
-    c.gscan_bytes = group_scan_bytes(mn);
-    TPM_HIP(hipMalloc(&c.gscan_tmp, c.gscan_bytes ? c.gscan_bytes : 16));
     TPM_HIP(hipMalloc(&c.ngroups_dev, 16));
     TPM_HIP(hipMalloc(&c.walk_ctr, 16 * sizeof(uint32_t)));
     c.dep_level = 0;
@@ -130,8 +129,6 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     {
         const size_t nodes1 = (size_t(1) << (3 * c.dep_level)) + 1;
         TPM_HIP(hipMalloc(&c.dep_start, 3 * nodes1 * sizeof(uint32_t)));
-        TPM_HIP(tpm::rocprim_exclusive_scan_u32(nullptr, c.dep_scan_bytes, nullptr, nullptr, nodes1, nullptr));
-        TPM_HIP(hipMalloc(&c.dep_scan_tmp, c.dep_scan_bytes ? c.dep_scan_bytes : 16));
     }
     TPM_HIP(hipMalloc(&c.counter, 8 * sizeof(unsigned long long)));
     TPM_HIP(hipMalloc(&c.flag, sizeof(int) * 4));
@@ -565,6 +562,20 @@ int tpm_judge_f64(tpm_ctx *ctx, const double *pos, size_t n, const uint32_t *idx
     });
 }
 
+int tpm_morton_order(tpm_ctx *ctx, const double *pos, size_t n, uint32_t *order, uint64_t *keys) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(n == 0 || order != nullptr, "order pointer is null");
+        tpm::upload_positions(c, pos, n, false);
+        tpm::sort_particles(c, c.pos, n);
+        if (n) {
+            TPM_HIP(hipMemcpyAsync(order, c.idx, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+            if (keys) TPM_HIP(hipMemcpyAsync(keys, c.keys, n * sizeof(uint64_t), hipMemcpyDeviceToHost, c.stream));
+        }
+        TPM_HIP(hipStreamSynchronize(c.stream));
+        return TPM_OK;
+    });
+}
+
 int tpm_get_device_status(tpm_ctx *ctx, uint32_t *status, int clear) {
     return guarded(ctx, [&](Ctx &c) {
         TPM_REQUIRE(status != nullptr, "status pointer is null");
@@ -813,8 +824,7 @@ int tpm_accelerations_learned_dev(tpm_ctx *ctx, const double *d_pos, size_t n, d
         }
         tm.mark();
         // Members = rejected sorted indices (exclusive scan + scatter).
-        size_t bytes = c.gscan_bytes;
-        TPM_HIP(tpm::rocprim_exclusive_scan_u32(c.gscan_tmp, bytes, c.reject, c.gscratch, n, c.stream));
+        tpm::exclusive_sum_u32(c, c.reject, c.gscratch, n);
         hipLaunchKernelGGL(k_compact, dim3(tpm::blocks(n, 256)), dim3(256), 0, c.stream, c.reject, c.gscratch, n,
                            c.members);
         uint32_t last[2];

@@ -121,8 +121,9 @@ struct Ctx {
     float *acc = nullptr;      // 3 max_n accelerations (caller order)
     uint64_t *keys = nullptr, *keys_alt = nullptr;
     uint32_t *idx = nullptr, *idx_alt = nullptr;
-    void *sort_tmp = nullptr;
-    size_t sort_tmp_bytes = 0;
+    uint32_t *radix_table = nullptr;  // per-tile digit counts of a radix pass (256 per 4096 keys)
+    size_t radix_table_words = 0;
+    uint32_t *scan_part = nullptr;    // partials of the device-wide scans (kScanParts words)
     double *spos = nullptr;    // 3 max_n positions in Morton order
     unsigned long long *counter = nullptr;  // interaction counter + scratch words
     int *flag = nullptr;       // non-finite flag
@@ -155,8 +156,6 @@ struct Ctx {
     // Walk groups: particles [gstart[g], gstart[g+1]) of the Morton order.
     uint32_t *gstart = nullptr;     // max_n + 1
     uint32_t *gscratch = nullptr;   // 2 max_n words
-    void *gscan_tmp = nullptr;
-    size_t gscan_bytes = 0;
     int group_level = 2;
     int group_level_shift = 0;  // autotuned offset of the group cube level
     int fft_cap = 16;           // autotuned cap on FFT lines per workgroup (LDS line kernels)
@@ -173,8 +172,6 @@ struct Ctx {
     uint32_t *ngroups_dev = nullptr;  // walk group count (device)
     uint32_t *walk_ctr = nullptr;     // walk group counters, one per XCD range (16 words)
     uint32_t *dep_start = nullptr;    // per deposit cube: first sorted index, item count, first item (3 (8^dep_level + 1))
-    void *dep_scan_tmp = nullptr;
-    size_t dep_scan_bytes = 0;
     int dep_level = 0;                // Morton level of the deposit cubes (at most 8 cells a side)
 };
 
@@ -216,10 +213,13 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only = false);
 // Walk groups: runs of at most 64 sorted particles that never cross a cube of octree level
 // c.group_level (sets c.gstart, c.ngroups).
 void build_groups(Ctx &c, size_t n, const uint64_t *keys = nullptr);
-size_t group_scan_bytes(size_t max_n);
-// Exclusive sum of n u32 values (rocPRIM); tmp = nullptr queries the temporary size.
-hipError_t rocprim_exclusive_scan_u32(void *tmp, size_t &bytes, const uint32_t *in, uint32_t *out, size_t n,
-                                      hipStream_t s);
+// Exclusive sum of n u32 values on c.stream (in == out allowed).
+void exclusive_sum_u32(Ctx &c, const uint32_t *in, uint32_t *out, size_t n);
+// Stable Morton sort of positions by the key digits of levels 1..depth (sort_kernels.hip): c.idx
+// (sorted slot -> caller index), c.spos (positions in sorted order) and c.keys (their keys; for
+// depth <= 10 with the levels below 10 cleared).
+size_t radix_tiles(size_t n);
+void morton_sort_positions(Ctx &c, const double *d_pos, size_t n, int depth);
 
 // S(u) table of the group walk for the context's r_split and r_cut.
 void make_short_range_table(Ctx &c);

@@ -14,83 +14,11 @@
 #include <cmath>
 #include <cstring>
 
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
-#include <rocprim/iterator/counting_iterator.hpp>
-#include <rocprim/iterator/reverse_iterator.hpp>
-#include <rocprim/iterator/transform_iterator.hpp>
-
+#include "device_scan.h"
+#include "morton_keys.h"
 #include "tpm_internal.h"
 
 namespace tpm {
-
-__device__ __forceinline__ uint64_t spread3(uint32_t v) {
-    uint64_t x = v & 0x1fffffu;
-    x = (x | x << 32) & 0x1f00000000ffffULL;
-    x = (x | x << 16) & 0x1f0000ff0000ffULL;
-    x = (x | x << 8) & 0x100f00f00f00f00fULL;
-    x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
-    x = (x | x << 2) & 0x1249249249249249ULL;
-    return x;
-}
-
-// 63-bit Morton key of a position: floor(x / L * 2^21) per axis, the division first, so a position
-// exactly on a dyadic plane of the box maps to the upper cell, like the reference's `p >= centre`
-// octant test.
-__device__ __forceinline__ uint64_t morton_key(const double *p, double box) {
-    const double q0 = floor((p[0] / box) * (double)(1u << kKeyBits));
-    const double q1 = floor((p[1] / box) * (double)(1u << kKeyBits));
-    const double q2 = floor((p[2] / box) * (double)(1u << kKeyBits));
-    const double qmax = (double)((1u << kKeyBits) - 1);
-    const uint32_t a = (uint32_t)fmin(fmax(q0, 0.0), qmax);
-    const uint32_t b = (uint32_t)fmin(fmax(q1, 0.0), qmax);
-    const uint32_t c = (uint32_t)fmin(fmax(q2, 0.0), qmax);
-    return spread3(a) | (spread3(b) << 1) | (spread3(c) << 2);
-}
-
-__global__ void k_morton(const double *__restrict__ pos, size_t n, double box, uint64_t *__restrict__ keys,
-                         uint32_t *__restrict__ idx) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    keys[i] = morton_key(pos + 3 * i, box);
-    idx[i] = (uint32_t)i;
-}
-
-// The top 30 bits of the key (octree levels 1-10), for a sort of 32-bit keys.
-constexpr int kHiShift = 3 * kKeyBits - 30;
-
-__global__ void k_morton_hi(const double *__restrict__ pos, size_t n, double box, uint32_t *__restrict__ keys,
-                            uint32_t *__restrict__ idx) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    keys[i] = (uint32_t)(morton_key(pos + 3 * i, box) >> kHiShift);
-    idx[i] = (uint32_t)i;
-}
-
-// Positions in sorted order; with `keys`, also the full keys of the sorted particles. Two
-// particles per thread (256 apart), their loads in flight together. Each XCD takes a contiguous
-// run of the sorted order (a compact region of the box), so the lines of input positions that
-// neighbouring cubes share are found in its L2.
-__global__ void __launch_bounds__(256) k_gather(const double *__restrict__ pos, const uint32_t *__restrict__ idx, size_t n,
-                                                double box, double *__restrict__ spos, uint64_t *__restrict__ keys) {
-    const size_t base = (size_t)xcd_block(blockIdx.x, gridDim.x) * 512 + threadIdx.x;
-    if (base >= n) return;
-    const bool two = base + 256 < n;
-    const size_t i1 = two ? base + 256 : base;
-    const size_t j0 = idx[base], j1 = idx[i1];
-    const double p0[3] = {pos[3 * j0], pos[3 * j0 + 1], pos[3 * j0 + 2]};
-    const double p1[3] = {pos[3 * j1], pos[3 * j1 + 1], pos[3 * j1 + 2]};
-    spos[3 * base] = p0[0];
-    spos[3 * base + 1] = p0[1];
-    spos[3 * base + 2] = p0[2];
-    if (keys) keys[base] = morton_key(p0, box);
-    if (two) {
-        spos[3 * i1] = p1[0];
-        spos[3 * i1 + 1] = p1[1];
-        spos[3 * i1 + 2] = p1[2];
-        if (keys) keys[i1] = morton_key(p1, box);
-    }
-}
 
 // ---------------------------------------------------------------------------- tree build
 
@@ -382,38 +310,9 @@ int sort_depth(const Ctx &c, size_t n) {
 }
 
 void sort_particles(Ctx &c, const double *d_pos, size_t n) {
-    if (n == 0) return;
-    size_t bytes = c.sort_tmp_bytes;
     // Only the key bits of the octree's levels are sorted (a shallower tree sorts fewer bits; the
     // sort is stable, so the order below that depth is the input order).
-    const int depth = sort_depth(c, n);
-    if (depth <= 10) {
-        // At most 30 key bits: 32-bit keys halve the key traffic of the radix passes; the gather
-        // recomputes the full keys of the sorted particles.
-        uint32_t *k32 = reinterpret_cast<uint32_t *>(c.keys), *k32_alt = reinterpret_cast<uint32_t *>(c.keys_alt);
-        hipLaunchKernelGGL(k_morton_hi, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_pos, n, c.p.box_size, k32,
-                           c.idx);
-        TPM_HIP(hipGetLastError());
-        rocprim::double_buffer<uint32_t> kb(k32, k32_alt);
-        rocprim::double_buffer<uint32_t> vb(c.idx, c.idx_alt);
-        TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 30 - 3 * depth, 30, c.stream));
-        if (vb.current() != c.idx) std::swap(c.idx, c.idx_alt);
-        hipLaunchKernelGGL(k_gather, dim3(blocks(n, 512)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.p.box_size,
-                           c.spos, c.keys);
-        TPM_HIP(hipGetLastError());
-        return;
-    }
-    hipLaunchKernelGGL(k_morton, dim3(blocks(n, 256)), dim3(256), 0, c.stream, d_pos, n, c.p.box_size, c.keys, c.idx);
-    TPM_HIP(hipGetLastError());
-    rocprim::double_buffer<uint64_t> kb(c.keys, c.keys_alt);
-    rocprim::double_buffer<uint32_t> vb(c.idx, c.idx_alt);
-    TPM_HIP(rocprim::radix_sort_pairs(c.sort_tmp, bytes, kb, vb, n, 3 * (kKeyBits - depth), 3 * kKeyBits,
-                                      c.stream));
-    if (kb.current() != c.keys) std::swap(c.keys, c.keys_alt);
-    if (vb.current() != c.idx) std::swap(c.idx, c.idx_alt);
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n, 512)), dim3(256), 0, c.stream, d_pos, c.idx, n, c.p.box_size, c.spos,
-                       (uint64_t *)nullptr);
-    TPM_HIP(hipGetLastError());
+    morton_sort_positions(c, d_pos, n, sort_depth(c, n));
 }
 
 __global__ void k_cube_starts(const uint64_t *__restrict__ keys, uint32_t n, int shift, uint32_t cubes,
@@ -432,17 +331,17 @@ __global__ void k_cube_mark(const uint64_t *__restrict__ keys, uint32_t n, int s
     if (i == 0 || (keys[i - 1] >> shift) != c) tab[c] = i;
 }
 
-using RevIter = rocprim::reverse_iterator<uint32_t *>;
-
-// Temporary storage of the suffix-minimum scan over up to `count` table entries.
-size_t cube_table_scan_bytes(size_t count) {
-    size_t bytes = 0;
-    uint32_t *p = nullptr;
-    if (rocprim::inclusive_scan(nullptr, bytes, RevIter(p + count), RevIter(p + count), count,
-                                rocprim::minimum<uint32_t>()) != hipSuccess)
-        throw Error{TPM_E_DEVICE, "scan size query failed"};
-    return bytes;
-}
+// The table read back to front (suffix minima are prefix minima of the reversed table).
+struct ReversedLoad {
+    const uint32_t *p;
+    size_t count;
+    __device__ __forceinline__ uint32_t operator()(size_t i) const { return p[count - 1 - i]; }
+};
+struct ReversedStore {
+    uint32_t *p;
+    size_t count;
+    __device__ __forceinline__ void operator()(size_t i, uint32_t v) const { p[count - 1 - i] = v; }
+};
 
 // Tree arrays for up to `cap` nodes (allocated once per capacity; never resized during a build).
 static void ensure_tree(Ctx &c, size_t cap) {
@@ -497,9 +396,8 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
         hipLaunchKernelGGL(k_cube_mark, dim3(blocks(n, 256)), dim3(256), 0, c.stream, c.keys, (uint32_t)n,
                            3 * (kKeyBits - tab_level), off);
         TPM_HIP(hipGetLastError());
-        size_t bytes = c.gscan_bytes;
-        TPM_HIP(rocprim::inclusive_scan(c.gscan_tmp, bytes, RevIter(off + entries), RevIter(tab + entries), entries,
-                                        rocprim::minimum<uint32_t>(), c.stream));
+        TPM_HIP(device_scan(ReversedLoad{off, entries}, ReversedStore{tab, entries}, entries, 0xFFFFFFFFu, ScanMin{},
+                            true, c.scan_part, c.stream));
     }
     // A level holds at most min(8^level, n) nodes (each holds a particle); the node-parallel kernels
     // get about one thread per possible node, so the deep levels' latency-bound loops (key loads,
@@ -534,15 +432,17 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
 struct GroupSegStart {
     const uint64_t *keys;
     int shift;
-    __device__ uint32_t operator()(uint32_t i) const {
-        return (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? i : 0u;
+    __device__ uint32_t operator()(size_t i) const {
+        return (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? (uint32_t)i : 0u;
     }
 };
 
 // Group starts: every kGroup-th particle of a cube, counted from the cube's first one.
 struct GroupChunkStart {
     const uint32_t *segstart;
-    __device__ uint32_t operator()(uint32_t i) const { return (i - segstart[i]) % (uint32_t)kGroup == 0 ? 1u : 0u; }
+    __device__ uint32_t operator()(size_t i) const {
+        return ((uint32_t)i - segstart[i]) % (uint32_t)kGroup == 0 ? 1u : 0u;
+    }
 };
 
 __global__ void k_group_starts(const uint32_t *__restrict__ segstart, const uint32_t *__restrict__ gid, size_t n,
@@ -583,10 +483,8 @@ struct CubeGroups {
     const uint32_t *starts;  // entry k << sh is cube k's first particle
     uint32_t cubes;
     int sh;
-    __device__ uint32_t operator()(uint32_t k) const {
-        return k < cubes ? (starts[(size_t)(k + 1) << sh] - starts[(size_t)k << sh] + (uint32_t)kGroup - 1) /
-                               (uint32_t)kGroup
-                         : 0u;
+    __device__ uint32_t operator()(size_t k) const {
+        return k < cubes ? (starts[(k + 1) << sh] - starts[k << sh] + (uint32_t)kGroup - 1) / (uint32_t)kGroup : 0u;
     }
 };
 
@@ -602,25 +500,6 @@ __global__ void k_cube_gstart(const uint32_t *__restrict__ starts, int sh, const
         gstart[gbase[cubes]] = starts[(size_t)cubes << sh];
         *ngroups = gbase[cubes];
     }
-}
-
-using SegIter = rocprim::transform_iterator<rocprim::counting_iterator<uint32_t>, GroupSegStart, uint32_t>;
-using CubeIter = rocprim::transform_iterator<rocprim::counting_iterator<uint32_t>, CubeGroups, uint32_t>;
-using ChunkIter = rocprim::transform_iterator<rocprim::counting_iterator<uint32_t>, GroupChunkStart, uint32_t>;
-
-size_t group_scan_bytes(size_t max_n) {
-    size_t a = 0, b = 0, e = 0;
-    uint32_t *p = nullptr;
-    const SegIter si(rocprim::counting_iterator<uint32_t>(0), GroupSegStart{nullptr, 0});
-    const ChunkIter ci(rocprim::counting_iterator<uint32_t>(0), GroupChunkStart{nullptr});
-    const CubeIter ki(rocprim::counting_iterator<uint32_t>(0), CubeGroups{nullptr, 0, 0});
-    size_t f = 0;
-    if (rocprim::inclusive_scan(nullptr, a, si, p, max_n, rocprim::maximum<uint32_t>()) != hipSuccess ||
-        rocprim::exclusive_scan(nullptr, b, ci, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess ||
-        rocprim::exclusive_scan(nullptr, e, p, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess ||
-        rocprim::exclusive_scan(nullptr, f, ki, p, 0u, max_n, rocprim::plus<uint32_t>()) != hipSuccess)
-        throw Error{TPM_E_DEVICE, "scan size query failed"};
-    return std::max(std::max(std::max(a, f), std::max(b, e)), cube_table_scan_bytes(max_n + 1));
 }
 
 void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
@@ -648,10 +527,8 @@ void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
             hipLaunchKernelGGL(k_cube_starts, dim3(blocks((size_t)cubes + 1, 256)), dim3(256), 0, c.stream, keys,
                                (uint32_t)n, shift, cubes, c.gscratch);
         }
-        size_t bytes = c.gscan_bytes;
-        const CubeIter ki(rocprim::counting_iterator<uint32_t>(0), CubeGroups{starts, cubes, sh});
-        TPM_HIP(rocprim::exclusive_scan(c.gscan_tmp, bytes, ki, gbase, 0u, (size_t)cubes + 1, rocprim::plus<uint32_t>(),
-                                        c.stream));
+        TPM_HIP(device_scan(CubeGroups{starts, cubes, sh}, ScanStore{gbase}, (size_t)cubes + 1, 0u, ScanSum{}, false,
+                            c.scan_part, c.stream));
         hipLaunchKernelGGL(k_cube_gstart, dim3(blocks(cubes, 256)), dim3(256), 0, c.stream, starts, sh, gbase, cubes,
                            c.gstart, c.ngroups_dev);
         TPM_HIP(hipGetLastError());
@@ -661,31 +538,12 @@ void build_groups(Ctx &c, size_t n, const uint64_t *keys) {
     uint32_t *segstart = c.gscratch + c.max_n;
     // First index of each particle's cube (max-scan of the cube starts), then the group number of
     // each particle (sum-scan of the group starts); both scans read their input through iterators.
-    size_t bytes = c.gscan_bytes;
-    const SegIter si(rocprim::counting_iterator<uint32_t>(0), GroupSegStart{keys, shift});
-    TPM_HIP(rocprim::inclusive_scan(c.gscan_tmp, bytes, si, segstart, n, rocprim::maximum<uint32_t>(), c.stream));
-    bytes = c.gscan_bytes;
-    const ChunkIter ci(rocprim::counting_iterator<uint32_t>(0), GroupChunkStart{segstart});
-    TPM_HIP(rocprim::exclusive_scan(c.gscan_tmp, bytes, ci, gid, 0u, n, rocprim::plus<uint32_t>(), c.stream));
+    TPM_HIP(device_scan(GroupSegStart{keys, shift}, ScanStore{segstart}, n, 0u, ScanMax{}, true, c.scan_part,
+                        c.stream));
+    TPM_HIP(device_scan(GroupChunkStart{segstart}, ScanStore{gid}, n, 0u, ScanSum{}, false, c.scan_part, c.stream));
     hipLaunchKernelGGL(k_group_starts, dim3(blocks(n, 256)), dim3(256), 0, c.stream, segstart, gid, n, c.gstart,
                        c.ngroups_dev);
     TPM_HIP(hipGetLastError());
-}
-
-hipError_t rocprim_exclusive_scan_u32(void *tmp, size_t &bytes, const uint32_t *in, uint32_t *out, size_t n,
-                                      hipStream_t s) {
-    return rocprim::exclusive_scan(tmp, bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), s);
-}
-
-size_t sort_temp_bytes(size_t max_n) {
-    size_t bytes = 0, bytes32 = 0;
-    rocprim::double_buffer<uint64_t> kb(nullptr, nullptr);
-    rocprim::double_buffer<uint32_t> kb32(nullptr, nullptr);
-    rocprim::double_buffer<uint32_t> vb(nullptr, nullptr);
-    if (rocprim::radix_sort_pairs(nullptr, bytes, kb, vb, max_n, 0, 3 * kKeyBits) != hipSuccess ||
-        rocprim::radix_sort_pairs(nullptr, bytes32, kb32, vb, max_n, 0, 30) != hipSuccess)
-        throw Error{TPM_E_DEVICE, "radix_sort_pairs size query failed"};
-    return std::max(bytes, bytes32);
 }
 
 }  // namespace tpm

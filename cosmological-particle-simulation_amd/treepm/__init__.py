@@ -54,6 +54,7 @@ EXPORTS = [
     "tpm_snapshot_dev", "tpm_run_settings_default", "tpm_run_settings_sanitized", "tpm_headless_run", "tpm_downsample_grid_dev", "tpm_autotune", "tpm_set_tuning", "tpm_ensemble_create", "tpm_ensemble_destroy", "tpm_ensemble_last_error",
     "tpm_ensemble_ics", "tpm_ensemble_autotune", "tpm_ensemble_run", "tpm_ensemble_get",
     "tpm_snapshot_write", "tpm_snapshot_read", "tpm_deposit_dev_f64_noalloc", "tpm_get_device_status",
+    "tpm_morton_order",
 ]
 
 
@@ -270,6 +271,7 @@ def load():
         "tpm_ensemble_get": (ctypes.c_int, [P, ctypes.c_int, P, P, P]),
         "tpm_deposit_dev_f64_noalloc": (ctypes.c_int, [P, S, ctypes.c_int32, D, D, P, P]),
         "tpm_get_device_status": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]),
+        "tpm_morton_order": (ctypes.c_int, [P, P, S, P, P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -528,6 +530,14 @@ class Solver:
 
     def sync(self):
         self._check(self.lib.tpm_sync(self.h))
+
+    def morton_order(self, pos):
+        """Octree (Morton) order of FP64 positions: (order (N,) uint32, sorted 63-bit keys (N,) uint64)."""
+        pos = _f64(pos, (-1, 3))
+        order = np.empty(len(pos), np.uint32)
+        keys = np.empty(len(pos), np.uint64)
+        self._check(self.lib.tpm_morton_order(self.h, _ptr(pos), len(pos), _ptr(order), _ptr(keys)))
+        return order, keys
 
     def device_status(self, clear=False):
         """Sticky device-side failure bits (STATUS_*) of the evaluations since the last clear."""

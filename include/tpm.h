@@ -130,6 +130,16 @@ int tpm_set_timing(tpm_ctx *ctx, int enabled);
  * saves one vector instruction per particle pair and tpm_stats.interactions reads 0. */
 int tpm_set_counting(tpm_ctx *ctx, int enabled);
 
+/*
+ * The particle order of the octree: Tree::build's counting sort into octants (src/tree.rs:215-236)
+ * applied at every level at once. order[k] is the caller index of the k-th particle in Morton
+ * order (octant digits bit 0 = x, bit 1 = y, bit 2 = z, per axis floor(x / L * 2^21)), sorted
+ * stably by the key digits of the context's octree depth (input order below it); keys (may be
+ * NULL) receives the sorted particles' 63-bit Morton keys (for a depth of at most 10, the digits of
+ * levels 11-21 read zero). Host memory.
+ */
+int tpm_morton_order(tpm_ctx *ctx, const double *pos, size_t n, uint32_t *order, uint64_t *keys);
+
 /* Mesh::deposit (src/mesh.rs:62-69): cells (n_mesh^3) = CIC deposit of `mass` per particle,
  * accumulated in 2^-32 fixed point (bit-exact run to run). */
 int tpm_deposit_f32(tpm_ctx *ctx, const float *pos, size_t n, float mass, float *cells);
