@@ -5,13 +5,14 @@
 // A run (initial force and `steps` KDK steps for every box) is captured once into a HIP graph:
 // the boxes' work is forked onto their own streams from one capture stream and joined back, so
 // one graph launch executes the whole ensemble with the boxes overlapping on the GPU. Every
-// operation of a force evaluation is a kernel launch (no host synchronisation, no memset), which
-// is what makes it capturable.
+// operation of a force evaluation is a kernel launch (no host synchronisation, no memset: the sort
+// and the scans are the library's own kernels), which is what makes it capturable.
 //
-// The autotuner times the candidates of two knobs on a box's own input and keeps the fastest:
-// the cap on FFT lines per workgroup (2, 4, 8, 16) and the octree level of the walk groups'
-// cubes (default, one level coarser, one finer). The default is always a candidate, so a tuned
-// box is never slower than the default on the timed input.
+// The autotuner times the candidates of its knobs on a box's own input and keeps the fastest
+// whose force error (against the FP64 judge at up to 2048 particles) stays within 10% of the
+// default's: the FFT kernels and lines per workgroup, the octree level of the walk groups' cubes,
+// the octree depth and the small-node rule. The default is always a candidate, so a tuned box is
+// never slower than the default on the timed input.
 #include <cmath>
 #include <memory>
 #include <string>
@@ -40,6 +41,7 @@ struct tpm_ensemble {
     hipGraphExec_t exec = nullptr;
     double g_a0 = 0, g_a1 = 0;
     int g_steps = -1;
+    int32_t g_nodes[3] = {0, 0, 0};  // kernel, memset and other nodes of the captured graph
     std::string err;
 };
 
@@ -48,8 +50,66 @@ tpm::Ctx &tpm_ctx_internal(tpm_ctx *ctx);
 
 namespace tpm {
 
+__global__ void k_gather_acc(const float *__restrict__ acc, const uint32_t *__restrict__ targets, size_t m,
+                             float *__restrict__ out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const size_t i = targets[t];
+    out[3 * t] = acc[3 * i];
+    out[3 * t + 1] = acc[3 * i + 1];
+    out[3 * t + 2] = acc[3 * i + 2];
+}
+
+// The accuracy yardstick of the tuner: the FP64 judge (theta 0.2) at up to 2048 evenly spaced
+// particles, against which each candidate's force error is measured (RMS of |a - a_judge| over the
+// RMS of |a_judge|).
+struct TuneRef {
+    uint32_t *targets = nullptr;  // device, m caller indices
+    float *picked = nullptr;      // device, 3 m
+    size_t m = 0;
+    std::vector<double> ref;      // host, 3 m
+    double norm2 = 0.0;
+    ~TuneRef() {
+        if (targets) (void)hipFree(targets);
+        if (picked) (void)hipFree(picked);
+    }
+};
+
+static void tune_reference(Ctx &c, const double *d_pos, size_t n, double a, TuneRef &r) {
+    r.m = std::min<size_t>(n, 2048);
+    std::vector<uint32_t> t(r.m);
+    for (size_t q = 0; q < r.m; ++q) t[q] = (uint32_t)((q * n) / r.m);
+    uint32_t *d_caller = nullptr;
+    double *d_out = nullptr;
+    judge_buffers(c, r.m, &d_caller, &d_out);
+    TPM_HIP(hipMemcpy(d_caller, t.data(), r.m * sizeof(uint32_t), hipMemcpyHostToDevice));
+    judge_dev(c, d_pos, n, d_caller, r.m, a, 0.2, d_out);
+    r.ref.resize(3 * r.m);
+    TPM_HIP(hipMemcpy(r.ref.data(), d_out, 3 * r.m * sizeof(double), hipMemcpyDeviceToHost));
+    r.norm2 = 0.0;
+    for (double v : r.ref) r.norm2 += v * v;
+    TPM_HIP(hipMalloc(&r.targets, r.m * sizeof(uint32_t)));
+    TPM_HIP(hipMalloc(&r.picked, 3 * r.m * sizeof(float)));
+    TPM_HIP(hipMemcpy(r.targets, t.data(), r.m * sizeof(uint32_t), hipMemcpyHostToDevice));
+}
+
+static double tune_error(Ctx &c, const float *d_acc, const TuneRef &r) {
+    hipLaunchKernelGGL(k_gather_acc, dim3((unsigned)((r.m + 255) / 256)), dim3(256), 0, c.stream, d_acc, r.targets,
+                       r.m, r.picked);
+    TPM_HIP(hipGetLastError());
+    std::vector<float> got(3 * r.m);
+    TPM_HIP(hipMemcpyAsync(got.data(), r.picked, 3 * r.m * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    TPM_HIP(hipStreamSynchronize(c.stream));
+    double e2 = 0.0;
+    for (size_t q = 0; q < 3 * r.m; ++q) {
+        const double d = (double)got[q] - r.ref[q];
+        e2 += d * d;
+    }
+    return r.norm2 > 0 ? std::sqrt(e2 / r.norm2) : 0.0;
+}
+
 static double tune_eval_ms(Ctx &c, const double *d_pos, size_t n, double a, float *d_acc, int reps, double *fft_ms,
-                           double *walk_ms) {
+                           double *walk_ms, const TuneRef *ref = nullptr, double *err = nullptr) {
     const bool saved = c.timing;
     c.timing = true;
     double best = 1e30, bf = 1e30, bw = 1e30;
@@ -62,6 +122,7 @@ static double tune_eval_ms(Ctx &c, const double *d_pos, size_t n, double a, floa
     c.timing = saved;
     if (fft_ms) *fft_ms = bf;
     if (walk_ms) *walk_ms = bw;
+    if (ref && err) *err = tune_error(c, d_acc, *ref);
     return best;
 }
 
@@ -69,8 +130,15 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
     tpm_tuning t{};
     const int cap0 = c.fft_cap, shift0 = c.group_level_shift, path0 = c.fft_path, depth0 = c.tree_depth;
     const int small0 = c.small_direct;
-    double f0, w0;
-    t.ms_default = tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f0, &w0);
+    // Candidates whose force error exceeds the default's by more than 10% are rejected, whatever
+    // their speed (the knobs below change which nodes the walk accepts and its summation order).
+    TuneRef ref;
+    tune_reference(c, d_pos, n, a, ref);
+    double f0, w0, e0 = 0.0;
+    t.ms_default = tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f0, &w0, &ref, &e0);
+    t.err_default = e0;
+    const double err_cap = 1.1 * e0 + 1e-7;
+    auto accurate = [&](double e) { return e <= err_cap; };
     int best_cap = cap0, best_path = path0;
     double best_f = f0;
     // FFT kernels: the wavefront transforms (mesh side 512 only) or the LDS line kernels with each
@@ -82,9 +150,9 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
             if (path == path0 && cap == cap0) continue;
             c.fft_path = path;
             c.fft_cap = cap;
-            double f, w;
-            tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f, &w);
-            if (f < best_f) {
+            double f, w, e;
+            tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f, &w, &ref, &e);
+            if (f < best_f && accurate(e)) {
                 best_f = f;
                 best_cap = cap;
                 best_path = path;
@@ -97,9 +165,9 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
     double best_w = w0;
     for (int sh : {-1, 1}) {
         c.group_level_shift = shift0 + sh;
-        double f, w;
-        tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f, &w);
-        if (w < best_w) {
+        double f, w, e;
+        tune_eval_ms(c, d_pos, n, a, d_acc, 2, &f, &w, &ref, &e);
+        if (w < best_w && accurate(e)) {
             best_w = w;
             best_shift = shift0 + sh;
         }
@@ -112,8 +180,9 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
     for (int depth : {16, 12, 10, 8}) {
         if (depth == depth0) continue;
         c.tree_depth = depth;
-        const double tt = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr);
-        if (tt < best_total) {
+        double e;
+        const double tt = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr, &ref, &e);
+        if (tt < best_total && accurate(e)) {
             best_total = tt;
             best_depth = depth;
         }
@@ -123,15 +192,19 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
     // nodes are mostly near, as in nearly uniform boxes), judged on the whole evaluation.
     int best_small = small0;
     c.small_direct = 1 - small0;
-    if (tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr) < best_total) best_small = 1 - small0;
+    {
+        double e;
+        if (tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr, &ref, &e) < best_total && accurate(e))
+            best_small = 1 - small0;
+    }
     c.small_direct = best_small;
     t.small_node_direct = best_small;
     t.tree_depth = best_depth;
     t.fft_cap = best_cap;
     t.fft_path = best_path;
     t.group_level_shift = best_shift;
-    t.ms_tuned = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr);
-    if (t.ms_tuned > t.ms_default) {
+    t.ms_tuned = tune_eval_ms(c, d_pos, n, a, d_acc, 2, nullptr, nullptr, &ref, &t.err_tuned);
+    if (t.ms_tuned > t.ms_default || !accurate(t.err_tuned)) {
         // Timing noise can make the tuned choice lose on the final check: keep the default then.
         c.fft_cap = cap0;
         c.fft_path = path0;
@@ -144,6 +217,7 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
         t.tree_depth = depth0;
         t.group_level_shift = shift0;
         t.ms_tuned = t.ms_default;
+        t.err_tuned = t.err_default;
     }
     return t;
 }
@@ -357,6 +431,24 @@ int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start
                 TPM_HIP(hipStreamEndCapture(E.master, &E.graph));
                 for (auto ev : joins) (void)hipEventDestroy(ev);
                 (void)hipEventDestroy(fork);
+                // Every operation of a run must be a kernel (memset nodes have given wrong results on
+                // replays after the first on these machines): count the node types and refuse a
+                // graph that holds a memset.
+                size_t count = 0;
+                TPM_HIP(hipGraphGetNodes(E.graph, nullptr, &count));
+                std::vector<hipGraphNode_t> nodes(count);
+                if (count) TPM_HIP(hipGraphGetNodes(E.graph, nodes.data(), &count));
+                E.g_nodes[0] = E.g_nodes[1] = E.g_nodes[2] = 0;
+                for (hipGraphNode_t nd : nodes) {
+                    hipGraphNodeType ty;
+                    TPM_HIP(hipGraphNodeGetType(nd, &ty));
+                    ++E.g_nodes[ty == hipGraphNodeTypeKernel ? 0 : ty == hipGraphNodeTypeMemset ? 1 : 2];
+                }
+                if (E.g_nodes[1] != 0) {
+                    (void)hipGraphDestroy(E.graph);
+                    E.graph = nullptr;
+                    throw tpm::Error{TPM_E_DEVICE, "the captured run holds memset nodes"};
+                }
                 TPM_HIP(hipGraphInstantiate(&E.exec, E.graph, nullptr, nullptr, 0));
                 E.g_a0 = a_start;
                 E.g_a1 = a_end;
@@ -373,6 +465,15 @@ int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start
         if (ms) *ms = t;
         (void)hipEventDestroy(t0);
         (void)hipEventDestroy(t1);
+        return TPM_OK;
+    });
+}
+
+int tpm_ensemble_graph_nodes(tpm_ensemble *e, int32_t *kernels, int32_t *memsets, int32_t *others) {
+    return eguard(e, [&](tpm_ensemble &E) {
+        if (kernels) *kernels = E.graph ? E.g_nodes[0] : 0;
+        if (memsets) *memsets = E.graph ? E.g_nodes[1] : 0;
+        if (others) *others = E.graph ? E.g_nodes[2] : 0;
         return TPM_OK;
     });
 }

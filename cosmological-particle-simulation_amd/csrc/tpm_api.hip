@@ -291,6 +291,62 @@ static int run_host_accel(Ctx &c, const void *pos, size_t n, double a, float *ac
     return TPM_OK;
 }
 
+// Judge scratch: m caller targets (u32) at *caller, then sorted targets and 6 m doubles (PM and
+// short-range parts); *out receives the 3 m doubles of the result.
+void judge_buffers(Ctx &c, size_t m, uint32_t **caller, double **out) {
+    if (!c.inv) TPM_HIP(hipMalloc(&c.inv, (c.max_n ? c.max_n : 1) * sizeof(uint32_t)));
+    if (c.judge_cap < m) {
+        if (c.judge_buf) (void)hipFree(c.judge_buf);
+        c.judge_buf = nullptr;
+        TPM_HIP(hipMalloc(&c.judge_buf, m * (sizeof(uint32_t) * 2 + 6 * sizeof(double)) + 256));
+        c.judge_cap = m;
+    }
+    *caller = (uint32_t *)c.judge_buf;
+    *out = (double *)((char *)c.judge_buf + ((m * 2 * sizeof(uint32_t) + 15) / 16) * 16);
+}
+
+// The FP64 judge on device positions for the caller targets d_caller (judge_buffers' layout), result
+// into d_out (3 m doubles, = judge_buffers' out). Synchronises: a target out of range throws.
+void judge_dev(Ctx &c, const double *d_pos, size_t n, const uint32_t *d_caller, size_t m, double a, double theta,
+               double *d_out) {
+    TPM_REQUIRE(n > 0 && n <= c.max_n, "judge needs 1..capacity particles");
+    TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
+    TPM_REQUIRE(theta >= 0 && std::isfinite(theta), "theta must be >= 0");
+    TPM_REQUIRE(m <= c.judge_cap, "judge buffers too small");
+    // The judge's octree is the full-depth one whatever the tuned depth limit (the yardstick must
+    // not depend on tuning); the product's next evaluation sorts and builds again.
+    struct DepthScope {
+        Ctx &c;
+        int saved;
+        ~DepthScope() { c.tree_depth = saved; }
+    } depth_scope{c, c.tree_depth};
+    c.tree_depth = kMaxDepth;
+    sort_particles(c, d_pos, n);
+    deposit_fixed(c, n, 1.0, false, c.spos, true);
+    const bool treepm = (c.p.flags & TPM_FLAG_TREEPM) != 0;
+    pm_from_grid_f64(c, n, a, treepm ? c.p.r_split : 0.0);
+    uint32_t *d_sorted = const_cast<uint32_t *>(d_caller) + m;
+    double *d_pm = d_out;
+    double *d_sr = d_pm + 3 * m;
+    hipLaunchKernelGGL(k_inverse_perm, dim3(blocks(n, 256)), dim3(256), 0, c.stream, c.idx, n, c.inv);
+    zero_fill(c, c.flag, sizeof(int));
+    hipLaunchKernelGGL(k_map_targets, dim3(blocks(m, 256)), dim3(256), 0, c.stream, c.inv, d_caller, m, n, d_sorted,
+                       c.flag);
+    TPM_HIP(hipGetLastError());
+    interp_judge_f64(c, d_sorted, m, d_pm);
+    if (treepm) {
+        build_tree(c, n);
+        const double k = 1.5 * c.p.omega_m * c.p.box_size * c.p.box_size * c.p.box_size / (4.0 * M_PI * a * (double)n);
+        walk_judge_f64(c, n, d_sorted, m, k, theta, d_sr);
+        hipLaunchKernelGGL(k_add3, dim3(blocks(3 * m, 256)), dim3(256), 0, c.stream, d_pm, d_sr, 3 * m);
+        TPM_HIP(hipGetLastError());
+    }
+    int bad = 0;
+    TPM_HIP(hipMemcpyAsync(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    TPM_HIP(hipStreamSynchronize(c.stream));
+    TPM_REQUIRE(!bad, "judge target index out of range");
+}
+
 }  // namespace tpm
 
 using tpm::Ctx;
@@ -514,50 +570,14 @@ int tpm_judge_f64(tpm_ctx *ctx, const double *pos, size_t n, const uint32_t *idx
                   double *acc) {
     return guarded(ctx, [&](Ctx &c) {
         TPM_REQUIRE(n > 0 && pos && idx && acc, "judge needs particles, targets and an output");
-        TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
-        TPM_REQUIRE(theta >= 0 && std::isfinite(theta), "theta must be >= 0");
         tpm::upload_positions(c, pos, n, false);
-        // The judge's octree is the full-depth one whatever the tuned depth limit (the yardstick
-        // must not depend on tuning); the product's next evaluation sorts and builds again.
-        struct DepthScope {
-            Ctx &c;
-            int saved;
-            ~DepthScope() { c.tree_depth = saved; }
-        } depth_scope{c, c.tree_depth};
-        c.tree_depth = tpm::kMaxDepth;
-        tpm::sort_particles(c, c.pos, n);
-        tpm::deposit_fixed(c, n, 1.0, false, c.spos, true);
-        const bool treepm = (c.p.flags & TPM_FLAG_TREEPM) != 0;
-        tpm::pm_from_grid_f64(c, n, a, treepm ? c.p.r_split : 0.0);
-        if (!c.inv) TPM_HIP(hipMalloc(&c.inv, (c.max_n ? c.max_n : 1) * sizeof(uint32_t)));
-        if (c.judge_cap < m) {
-            if (c.judge_buf) (void)hipFree(c.judge_buf);
-            TPM_HIP(hipMalloc(&c.judge_buf, m * (sizeof(uint32_t) * 2 + 6 * sizeof(double)) + 256));
-            c.judge_cap = m;
-        }
-        uint32_t *d_caller = (uint32_t *)c.judge_buf;
-        uint32_t *d_sorted = d_caller + m;
-        double *d_pm = (double *)((char *)c.judge_buf + ((m * 2 * sizeof(uint32_t) + 15) / 16) * 16);
-        double *d_sr = d_pm + 3 * m;
+        uint32_t *d_caller = nullptr;
+        double *d_out = nullptr;
+        tpm::judge_buffers(c, m, &d_caller, &d_out);
         TPM_HIP(hipMemcpyAsync(d_caller, idx, m * sizeof(uint32_t), hipMemcpyHostToDevice, c.stream));
-        hipLaunchKernelGGL(tpm::k_inverse_perm, dim3(tpm::blocks(n, 256)), dim3(256), 0, c.stream, c.idx, n, c.inv);
-        tpm::zero_fill(c, c.flag, sizeof(int));
-        hipLaunchKernelGGL(tpm::k_map_targets, dim3(tpm::blocks(m, 256)), dim3(256), 0, c.stream, c.inv, d_caller, m, n,
-                           d_sorted, c.flag);
-        TPM_HIP(hipGetLastError());
-        tpm::interp_judge_f64(c, d_sorted, m, d_pm);
-        if (treepm) {
-            tpm::build_tree(c, n);
-            const double k = 1.5 * c.p.omega_m * c.p.box_size * c.p.box_size * c.p.box_size / (4.0 * M_PI * a * (double)n);
-            tpm::walk_judge_f64(c, n, d_sorted, m, k, theta, d_sr);
-            hipLaunchKernelGGL(tpm::k_add3, dim3(tpm::blocks(3 * m, 256)), dim3(256), 0, c.stream, d_pm, d_sr, 3 * m);
-            TPM_HIP(hipGetLastError());
-        }
-        int bad = 0;
-        TPM_HIP(hipMemcpyAsync(&bad, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        TPM_HIP(hipMemcpyAsync(acc, d_pm, 3 * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        tpm::judge_dev(c, c.pos, n, d_caller, m, a, theta, d_out);
+        TPM_HIP(hipMemcpyAsync(acc, d_out, 3 * m * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         TPM_HIP(hipStreamSynchronize(c.stream));
-        TPM_REQUIRE(!bad, "judge target index out of range");
         return TPM_OK;
     });
 }

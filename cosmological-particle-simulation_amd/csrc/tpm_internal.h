@@ -247,6 +247,11 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
 // Enqueues the fold of this evaluation's failure flags (walk stack overflow, octree capacity when
 // the tree was built, non-finite accelerations when checked) into the sticky c.status word.
 void fold_status(Ctx &c, bool tree_built);
+// FP64 TreePM judge (FP64 PM + per-particle FP64 walk at theta on the full-depth octree) of the
+// caller-order targets d_caller (from judge_buffers) on device positions; 3 m doubles into d_out.
+void judge_buffers(Ctx &c, size_t m, uint32_t **caller, double **out);
+void judge_dev(Ctx &c, const double *d_pos, size_t n, const uint32_t *d_caller, size_t m, double a, double theta,
+               double *d_out);
 // Synchronises c.stream and throws on a walk stack overflow, octree capacity overflow or (with
 // TPM_FLAG_CHECK_FINITE) a non-finite acceleration of the last evaluation.
 void finish_checks(Ctx &c);

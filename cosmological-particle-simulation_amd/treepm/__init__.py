@@ -54,7 +54,7 @@ EXPORTS = [
     "tpm_snapshot_dev", "tpm_run_settings_default", "tpm_run_settings_sanitized", "tpm_headless_run", "tpm_downsample_grid_dev", "tpm_autotune", "tpm_set_tuning", "tpm_ensemble_create", "tpm_ensemble_destroy", "tpm_ensemble_last_error",
     "tpm_ensemble_ics", "tpm_ensemble_autotune", "tpm_ensemble_run", "tpm_ensemble_get",
     "tpm_snapshot_write", "tpm_snapshot_read", "tpm_deposit_dev_f64_noalloc", "tpm_get_device_status",
-    "tpm_morton_order",
+    "tpm_morton_order", "tpm_ensemble_graph_nodes",
 ]
 
 
@@ -136,7 +136,7 @@ class Stats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("fft_cap", ctypes.c_int32), ("group_level_shift", ctypes.c_int32), ("fft_path", ctypes.c_int32),
                 ("tree_depth", ctypes.c_int32), ("small_node_direct", ctypes.c_int32), ("ms_default", ctypes.c_double),
-                ("ms_tuned", ctypes.c_double)]
+                ("ms_tuned", ctypes.c_double), ("err_default", ctypes.c_double), ("err_tuned", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -272,6 +272,7 @@ def load():
         "tpm_deposit_dev_f64_noalloc": (ctypes.c_int, [P, S, ctypes.c_int32, D, D, P, P]),
         "tpm_get_device_status": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]),
         "tpm_morton_order": (ctypes.c_int, [P, P, S, P, P]),
+        "tpm_ensemble_graph_nodes": (ctypes.c_int, [P, P, P, P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -622,6 +623,12 @@ class Ensemble:
         self._check(self.lib.tpm_ensemble_run(self.h, ctypes.byref(cosmo), float(a_start), float(a_end), int(steps),
                                               1 if graph else 0, ctypes.byref(ms)))
         return ms.value
+
+    def graph_nodes(self):
+        """Node counts of the captured run's graph: {"kernels", "memsets", "others"}."""
+        v = [ctypes.c_int32() for _ in range(3)]
+        self._check(self.lib.tpm_ensemble_graph_nodes(self.h, *[ctypes.byref(x) for x in v]))
+        return {"kernels": v[0].value, "memsets": v[1].value, "others": v[2].value}
 
     def get(self, b):
         n = self.sides[b] ** 3

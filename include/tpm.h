@@ -381,10 +381,10 @@ int tpm_snapshot_read(const char *path, tpm_snapshot_header *header, double *pos
 
 /*
  * Autotuning (no reference counterpart): times the candidates of the knobs below on the given input
- * and keeps the fastest (the default is a candidate): the FFT kernels (wavefront transforms or LDS
- * line kernels), the cap on FFT lines per workgroup of the latter, the octree level of the walk
- * groups' cubes (as an offset from the default level), the octree depth limit and the small-node
- * rule.
+ * and keeps the fastest (the default is a candidate) among those whose force error against the FP64
+ * judge stays within 10% of the default's: the FFT kernels (wavefront transforms or LDS line
+ * kernels), the cap on FFT lines per workgroup of the latter, the octree level of the walk groups'
+ * cubes (as an offset from the default level), the octree depth limit and the small-node rule.
  */
 typedef struct tpm_tuning {
     int32_t fft_cap;
@@ -396,6 +396,9 @@ typedef struct tpm_tuning {
                                   monopoles; 1: they are summed directly too (more pairs, no monopole) */
     double ms_default; /* force evaluation with the default knobs */
     double ms_tuned;   /* with the chosen knobs */
+    double err_default; /* force error with the default knobs: RMS |a - a_judge| / RMS |a_judge| against the
+                           FP64 judge (theta 0.2) at up to 2048 evenly spaced particles */
+    double err_tuned;   /* the same with the chosen knobs (candidates above 1.1 err_default are rejected) */
 } tpm_tuning;
 
 int tpm_autotune(tpm_ctx *ctx, const double *d_pos, size_t n, double a, float *d_acc, tpm_tuning *out);
@@ -426,6 +429,10 @@ int tpm_ensemble_ics(tpm_ensemble *ens, const uint64_t *seeds, double k_c, doubl
 int tpm_ensemble_autotune(tpm_ensemble *ens, double a, tpm_tuning *out);
 int tpm_ensemble_run(tpm_ensemble *ens, const tpm_cosmology *cosmo, double a_start, double a_end, int steps,
                      int use_graph, double *ms);
+/* Node counts of the captured run's HIP graph by type (kernels, memsets, other nodes such as event
+ * records and waits); zeros before the first graph run. A capture that records a memset node is
+ * refused (TPM_E_DEVICE): every operation of a run is a kernel. */
+int tpm_ensemble_graph_nodes(tpm_ensemble *ens, int32_t *kernels, int32_t *memsets, int32_t *others);
 /* Copies box `box`'s current positions, momenta (3 side^3 doubles each) and accelerations (floats). */
 int tpm_ensemble_get(tpm_ensemble *ens, int box, double *pos, double *mom, float *acc);
 

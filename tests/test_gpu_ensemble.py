@@ -72,6 +72,8 @@ def test_autotune_keeps_results_and_is_not_slower(treepm):
         assert t["fft_cap"] in (2, 4, 8, 16) and t["fft_path"] in (0, 1) and t["tree_depth"] in (8, 10, 12, 16, 21)
         assert t["group_level_shift"] in (-1, 0, 1) and t["small_node_direct"] in (0, 1)
         assert t["ms_tuned"] <= t["ms_default"]
+        # the accuracy guard: the tuned knobs' error against the FP64 judge is within 10% of the default's
+        assert 0 < t["err_default"] < 0.1 and t["err_tuned"] <= 1.1 * t["err_default"] + 1e-7
     e.run(A0, A1, 0, graph=True)
     got = _states(e)
     for b in range(len(SIDES)):
@@ -141,3 +143,24 @@ def test_solver_autotune_and_set_tuning(treepm):
     assert t.ms_tuned <= t.ms_default
     with pytest.raises(treepm.TpmError):
         s.set_tuning(0, 0)
+
+
+def test_graph_of_a_large_box_has_no_memset_nodes(treepm):
+    # A box of 128^3 = 2,097,152 particles: the sort, the scans and every other operation of a run
+    # are kernels, so the captured graph holds no memset node, and its replays equal the eager run.
+    sides, seeds = [128, 32], [31, 32]
+    print(f"[gpu] ensemble sides={sides}, graph node types", flush=True)
+    e = treepm.Ensemble(sides, box_size=100.0, omega_m=OMEGA_M, softening_scale=1 / 30, theta=0.5)
+    e.ics(seeds, k_c=2.0, disp_cells=0.3, mom_factor=MOM)
+    e.run(A0, A1, 2, graph=False)
+    eager = [e.get(b) for b in range(len(sides))]
+    for rep in range(3):
+        e.run(A0, A1, 2, graph=True)
+        nodes = e.graph_nodes()
+        print(f"[gpu] graph replay {rep}: {nodes}", flush=True)
+        assert nodes["memsets"] == 0 and nodes["kernels"] > 100
+        for b in range(len(sides)):
+            for x, y in zip(e.get(b), eager[b]):
+                assert np.array_equal(x, y), (rep, b)
+        e.run(A0, A1, 1, graph=False)  # eager work between replays
+    e.close()
