@@ -3,9 +3,11 @@
 // and backward kernels. No reference counterpart (the reference has only the exact tree walk);
 // the walk supplies its training targets, its fallback and the judge of its error.
 //
-// Per particle the input is 33 features, zero-padded to 64: log(1 + delta) on the 3x3x3 cells
-// around the particle's cell, the PM force at the particle and the particle's offset from its
-// cell centre (cell units). Forces enter and leave in the compressed units slog(F / (K / h^2)),
+// Per particle the input is 33 features, zero-padded to 64: log(1 + delta) of the particle's cell and,
+// relative to it, of the 26 cells around it (the 3x3x3 stencil; differences keep the gradients that
+// BF16 would round away on top of a common level), the PM force at the particle and the particle's
+// offset from its cell centre (cell units). The inference runs all of it as one kernel
+// (k_mlp_fused, below the training kernels). Forces enter and leave in the compressed units slog(F / (K / h^2)),
 // slog(x) = sign(x) log(1 + |x|), since short-range forces in dense regions span many decades.
 // Outputs: the short-range force (3 components, compressed units) and s = log(predicted squared
 // error in those units). The force is trained by mean squared error and s by the
@@ -19,7 +21,9 @@
 // v_mfma_f32_32x32x16_bf16 over K. Lane l supplies row (l mod 32) of A and row (l mod 32) of Bt,
 // elements k0 + 8 (l / 32) .. + 7; it receives C rows 8 g + 4 (l / 32) + r (g, r < 4) of column
 // (l mod 32).
+#include <algorithm>
 #include <cmath>
+#include <map>
 #include <vector>
 
 #include "tpm_internal.h"
@@ -29,12 +33,17 @@ namespace tpm {
 typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
-// Round-to-nearest-even FP32 -> BF16 (finite inputs).
+// Round-to-nearest-even FP32 -> BF16 (finite inputs): the hardware conversion on the device
+// (v_cvt_pk_bf16_f32), the same rounding in integer arithmetic on the host.
 __host__ __device__ inline uint16_t to_bf16(float x) {
+#if __HIP_DEVICE_COMPILE__
+    return __builtin_bit_cast(uint16_t, (__bf16)x);
+#else
     uint32_t u;
     memcpy(&u, &x, 4);
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
+#endif
 }
 
 __host__ __device__ inline float from_bf16(uint16_t b) {
@@ -68,9 +77,12 @@ __global__ void __launch_bounds__(64) k_gemm_bf16(const uint16_t *__restrict__ A
 __device__ inline float slog(float x) { return copysignf(log1pf(fabsf(x)), x); }
 __device__ inline float sexp(float y) { return copysignf(expm1f(fabsf(y)), y); }
 
-__device__ inline float silu(float z) { return z / (1.f + __expf(-z)); }
+// SiLU z sigmoid(z), the sigmoid by the hardware exponential and reciprocal (about 1 ulp each; the
+// result is rounded to BF16 next).
+__device__ inline float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigmoid(z); }
 __device__ inline float silu_grad(float z) {
-    const float s = 1.f / (1.f + __expf(-z));
+    const float s = sigmoid(z);
     return s * (1.f + z * (1.f - s));
 }
 
@@ -204,54 +216,80 @@ __global__ void k_to_bf16_both(const float *__restrict__ Wt, int n, int k, uint1
     W16[(size_t)col * n + row] = b;
 }
 
-// Features of sorted particles: 27 log(1 + delta) cells, PM force / fscale, offset in the cell.
+// Periodic wrap of a cell index; one compare-and-add for the indices of positions in [0, L]
+// (at most one mesh side out), the modulo otherwise.
+__device__ __forceinline__ int wrap_mesh(int v, int n) {
+    if (v >= 0 && v < n) return v;
+    if (v < 0 && v >= -n) return v + n;
+    if (v >= n && v < 2 * n) return v - n;
+    const int r = v % n;
+    return r < 0 ? r + n : r;
+}
+
+// The 33 features of sorted particle i: log(1 + delta) on the 3x3x3 cells around its cell (x, y, z
+// loops outer to inner; the 26 neighbours minus the particle's own cell; stencil_features), then the PM force interpolated at it (CIC, FP64 stencil) in
+// compressed units and its offset from the cell centre in cells (pm_offset_features).
+__device__ inline void stencil_features(const double *__restrict__ spos, size_t i, const float *__restrict__ delta,
+                                        int n, double box, float f[27]) {
+    const double h = box / n;
+    int c[3];
+    for (int a = 0; a < 3; ++a) c[a] = (int)floor(spos[3 * i + a] / h);
+    int q = 0;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int ix = wrap_mesh(c[0] + dx, n), iy = wrap_mesh(c[1] + dy, n), iz = wrap_mesh(c[2] + dz, n);
+                const float d = delta[((size_t)ix * n + iy) * n + iz];
+                f[q++] = __logf(fmaxf(1.f + d, 1e-3f));
+            }
+    // The neighbours relative to the particle's own cell (f[13]): the differences carry the gradients,
+    // which BF16 would lose on top of a common level.
+    for (int k = 0; k < 27; ++k)
+        if (k != 13) f[k] -= f[13];
+}
+
+__device__ inline void pm_offset_features(const double *__restrict__ spos, size_t i, const float *__restrict__ force,
+                                          int n, double box, float inv_fscale, float f[6]) {
+    const double h = box / n;
+    const double s0 = spos[3 * i] / h - 0.5, s1 = spos[3 * i + 1] / h - 0.5, s2 = spos[3 * i + 2] / h - 0.5;
+    const double b0 = floor(s0), b1 = floor(s1), b2 = floor(s2);
+    const double f0 = s0 - b0, f1 = s1 - b1, f2 = s2 - b2;
+    float g[3] = {0.f, 0.f, 0.f};
+    for (int dx = 0; dx < 2; ++dx)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dz = 0; dz < 2; ++dz) {
+                const double w = (dx ? f0 : 1 - f0) * (dy ? f1 : 1 - f1) * (dz ? f2 : 1 - f2);
+                const int ix = wrap_mesh((int)b0 + dx, n), iy = wrap_mesh((int)b1 + dy, n),
+                          iz = wrap_mesh((int)b2 + dz, n);
+                const size_t cell = ((size_t)ix * n + iy) * n + iz;
+                for (int a = 0; a < 3; ++a) g[a] += (float)w * force[3 * cell + a];
+            }
+    for (int a = 0; a < 3; ++a) f[a] = slog(g[a] * inv_fscale);
+    for (int a = 0; a < 3; ++a) {
+        const double sc = spos[3 * i + a] / h;
+        f[3 + a] = (float)(sc - floor(sc) - 0.5);
+    }
+}
+
+__device__ inline void particle_features(const double *__restrict__ spos, size_t i, const float *__restrict__ delta,
+                                         const float *__restrict__ force, int n, double box, float inv_fscale,
+                                         float f[33]) {
+    stencil_features(spos, i, delta, n, box, f);
+    pm_offset_features(spos, i, force, n, box, inv_fscale, f + 27);
+}
+
+// Features of sorted particles (rows of X, BF16, zero padded to ldx).
 __global__ void k_features(const double *__restrict__ spos, const uint32_t *__restrict__ sel, size_t first,
                            size_t count, const float *__restrict__ delta, const float *__restrict__ force, int n,
                            double box, size_t m, float inv_fscale, uint16_t *__restrict__ X, int ldx) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const size_t i = sel ? sel[t] : first + t;
-    const double h = box / n;
-    int c[3];
-    float off[3];
-    for (int a = 0; a < 3; ++a) {
-        const double s = spos[3 * i + a] / h;
-        const double f = floor(s);
-        c[a] = (int)f;
-        off[a] = (float)(s - f - 0.5);
-    }
+    float f[33];
+    particle_features(spos, i, delta, force, n, box, inv_fscale, f);
     uint16_t *x = X + t * (size_t)ldx;
     int q = 0;
-    for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dz = -1; dz <= 1; ++dz) {
-                int ix = (c[0] + dx) % n, iy = (c[1] + dy) % n, iz = (c[2] + dz) % n;
-                ix += ix < 0 ? n : 0;
-                iy += iy < 0 ? n : 0;
-                iz += iz < 0 ? n : 0;
-                const float d = delta[((size_t)ix * n + iy) * n + iz];
-                x[q++] = to_bf16(logf(fmaxf(1.f + d, 1e-3f)));
-            }
-    // PM force interpolated at the particle (CIC, FP64 stencil), scaled.
-    {
-        const double s0 = spos[3 * i] / h - 0.5, s1 = spos[3 * i + 1] / h - 0.5, s2 = spos[3 * i + 2] / h - 0.5;
-        const double b0 = floor(s0), b1 = floor(s1), b2 = floor(s2);
-        const double f0 = s0 - b0, f1 = s1 - b1, f2 = s2 - b2;
-        float g[3] = {0.f, 0.f, 0.f};
-        for (int dx = 0; dx < 2; ++dx)
-            for (int dy = 0; dy < 2; ++dy)
-                for (int dz = 0; dz < 2; ++dz) {
-                    const double w = (dx ? f0 : 1 - f0) * (dy ? f1 : 1 - f1) * (dz ? f2 : 1 - f2);
-                    int ix = ((int)b0 + dx) % n, iy = ((int)b1 + dy) % n, iz = ((int)b2 + dz) % n;
-                    ix += ix < 0 ? n : 0;
-                    iy += iy < 0 ? n : 0;
-                    iz += iz < 0 ? n : 0;
-                    const size_t cell = ((size_t)ix * n + iy) * n + iz;
-                    for (int a = 0; a < 3; ++a) g[a] += (float)w * force[3 * cell + a];
-                }
-        for (int a = 0; a < 3; ++a) x[q++] = to_bf16(slog(g[a] * inv_fscale));
-    }
-    for (int a = 0; a < 3; ++a) x[q++] = to_bf16(off[a]);
+    for (; q < 33; ++q) x[q] = to_bf16(f[q]);
     for (; q < ldx; ++q) x[q] = 0;
 }
 
@@ -286,6 +324,225 @@ __global__ void k_apply(const float *__restrict__ Zout, int nout, size_t count, 
         acc[3 * o + 2] = az;
     }
     reject[i] = ok ? 0u : 1u;
+}
+
+// ------------------------------------------------------------- fused inference (one kernel)
+//
+// Features, the four layers and the gate of every particle in one kernel, the weights resident in
+// LDS. Layers are computed transposed, C[out x particle] = Wt[out x in] * X[particle x in]^T: the
+// weights are the MFMA's A operand and the activations its B operand. A wavefront takes 32 particles
+// at a time; lane l works on particle l mod 32, supplies elements k0 + 8 (l / 32) .. + 7 of its row
+// and receives C rows 8 g + 4 (l / 32) + r of the same particle. After bias and SiLU, lanes l and
+// l + 32 exchange half their rows (v_permlane32_swap), after which each holds 8 consecutive
+// outputs of its particle and stores them as one 16-byte row segment of the wavefront's activation
+// block in LDS, where the next layer reads its operand. The arithmetic is that of mlp_forward (FP32
+// accumulation over k in the same order, bias and SiLU in FP32, BF16 between layers).
+constexpr int kFusedWaves = 8;
+// LDS rows of BF16 are stored in 16-byte chunks, chunk c of row r at position c ^ (r mod chunks), so
+// the 32 rows a fragment read touches spread over the banks without padding.
+constexpr int kLdsW0 = 0, kLdsW1 = kLdsW0 + 128 * 64, kLdsW2 = kLdsW1 + 128 * 128, kLdsW3 = kLdsW2 + 128 * 128,
+              kLdsWEnd = kLdsW3 + 32 * 128;
+constexpr int kLdsAct = kLdsWEnd;                              // per wavefront: 32 rows of 128
+constexpr int kLdsActEnd = kLdsAct + kFusedWaves * 32 * 128;
+
+// Element offset of (row, chunk) in a swizzled block with `chunks` 16-byte chunks per row.
+template <int CHUNKS>
+__device__ __forceinline__ int swz(int row, int chunk) {
+    return (row * CHUNKS + (chunk ^ (row & (CHUNKS - 1)))) * 8;
+}
+
+struct FusedArgs {
+    const double *spos;
+    const uint32_t *idx;     // sorted -> caller order
+    const float *delta, *force;
+    int n;
+    double box;
+    const uint16_t *wt[4];   // BF16 Wt [out x in] per layer
+    const float *bias[4];    // FP32 biases
+    const uint32_t *sel;     // sorted indices of the particles, or null: [first, first + count)
+    size_t first, count;
+    float fscale, inv_fscale, tol, kappa;
+    float *acc;              // caller order: += the prediction where accepted
+    uint32_t *reject;        // sorted order: 1 = walk it
+    float *pred;             // calibration mode (non-null): 4 floats per particle (force, sigma), no gate
+    const uint16_t *rows;    // rows mode (non-null): BF16 feature rows [count x 64] instead of particles;
+    float *zout;             // the four raw outputs per row go to zout
+};
+
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+    return (uint32_t)to_bf16(a) | ((uint32_t)to_bf16(b) << 16);
+}
+
+// One layer of the wavefront's 32 particles: the operand comes from the activation block `act` (NK
+// k-steps); the NM output blocks' accumulation chains run interleaved. With ACT the outputs go back
+// into `act` (bias, SiLU, BF16, lane exchange); without, block 0's first four rows plus bias go to z.
+template <int NM, int NK, bool ACT>
+__device__ __forceinline__ void fused_layer(const uint16_t *__restrict__ w, const float *__restrict__ bias,
+                                            uint16_t *act, float *z, int r32, int half) {
+    bf16x8 xin[NK];
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks) xin[ks] = *reinterpret_cast<const bf16x8 *>(act + swz<16>(r32, 2 * ks + half));
+    f32x16 acc[NM];
+#pragma unroll
+    for (int mt = 0; mt < NM; ++mt)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[mt][q] = 0.f;
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks)
+#pragma unroll
+        for (int mt = 0; mt < NM; ++mt) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(w + swz<NK * 2>(mt * 32 + r32, 2 * ks + half));
+            acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xin[ks], acc[mt], 0, 0, 0);
+        }
+    if (ACT) {
+#pragma unroll
+        for (int mt = 0; mt < NM; ++mt)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                // rows 16 p + 0..15 of block mt: groups g = 2p and 2p + 1
+                float v[2][4];
+#pragma unroll
+                for (int gg = 0; gg < 2; ++gg)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int g = 2 * p + gg;
+                        v[gg][r] = silu(acc[mt][4 * g + r] + bias[mt * 32 + 8 * g + 4 * half + r]);
+                    }
+                const uint32_t a0 = pack_bf16(v[0][0], v[0][1]), a1 = pack_bf16(v[0][2], v[0][3]);
+                const uint32_t b0 = pack_bf16(v[1][0], v[1][1]), b1 = pack_bf16(v[1][2], v[1][3]);
+                const auto s0 = __builtin_amdgcn_permlane32_swap(a0, b0, false, false);
+                const auto s1 = __builtin_amdgcn_permlane32_swap(a1, b1, false, false);
+                // lower lane: outputs 32 mt + 16 p + 0..7 (chunk 4 mt + 2 p); upper lane: + 8..15
+                uint4 seg;
+                seg.x = s0[0];
+                seg.y = s1[0];
+                seg.z = s0[1];
+                seg.w = s1[1];
+                *reinterpret_cast<uint4 *>(act + swz<16>(r32, 4 * mt + 2 * p + half)) = seg;
+            }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) z[r] = acc[0][r] + bias[4 * half + r];
+    }
+}
+
+__global__ void __launch_bounds__(64 * kFusedWaves) k_mlp_fused(FusedArgs A) {
+    extern __shared__ __align__(16) uint16_t lds[];
+    float *lbias = reinterpret_cast<float *>(lds + kLdsActEnd);  // 128 + 128 + 128 + 32
+    // Weights (swizzled rows) and biases into LDS.
+    {
+        const int nin[4] = {64, 128, 128, 128}, nout[4] = {128, 128, 128, 32};
+        const int base[4] = {kLdsW0, kLdsW1, kLdsW2, kLdsW3};
+        for (int l = 0; l < 4; ++l) {
+            const int chunks = nin[l] / 8, total = nout[l] * chunks;
+            const uint4 *src = reinterpret_cast<const uint4 *>(A.wt[l]);
+            for (int t = threadIdx.x; t < total; t += blockDim.x) {
+                const int row = t / chunks, ch = t % chunks;
+                *reinterpret_cast<uint4 *>(lds + base[l] + (row * chunks + (ch ^ (row & (chunks - 1)))) * 8) = src[t];
+            }
+        }
+        for (int t = threadIdx.x; t < 416; t += blockDim.x) {
+            const int l = t < 128 ? 0 : t < 256 ? 1 : t < 384 ? 2 : 3;
+            lbias[t] = A.bias[l][t - 128 * l];
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r32 = lane & 31, half = lane >> 5;
+    uint16_t *act = lds + kLdsAct + wave * 32 * 128;
+    const size_t tiles = (A.count + 31) / 32;
+    for (size_t tile = (size_t)blockIdx.x * kFusedWaves + wave; tile < tiles; tile += (size_t)gridDim.x * kFusedWaves) {
+        const size_t t = tile * 32 + r32;
+        const bool live = t < A.count;
+        const size_t tt = live ? t : A.count - 1;
+        const size_t i = A.sel ? A.sel[tt] : A.first + tt;
+        if (A.rows) {
+            // Given feature rows: the lower lane copies chunks 0-3, the upper 4-7.
+            const uint4 *src = reinterpret_cast<const uint4 *>(A.rows + tt * 64);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                *reinterpret_cast<uint4 *>(act + swz<16>(r32, 4 * half + q)) = src[4 * half + q];
+        } else if (!half) {
+            // Features 0-26 (the density stencil) by the lower lane ...
+            float f[27];
+            stencil_features(A.spos, i, A.delta, A.n, A.box, f);
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                *reinterpret_cast<uint4 *>(act + swz<16>(r32, q)) =
+                    make_uint4(pack_bf16(f[8 * q], f[8 * q + 1]), pack_bf16(f[8 * q + 2], f[8 * q + 3]),
+                               pack_bf16(f[8 * q + 4], f[8 * q + 5]), pack_bf16(f[8 * q + 6], f[8 * q + 7]));
+            uint16_t *c3 = act + swz<16>(r32, 3);
+            c3[0] = to_bf16(f[24]);
+            c3[1] = to_bf16(f[25]);
+            c3[2] = to_bf16(f[26]);
+        } else {
+            // ... 27-32 (PM force, offset) and the zero padding by the upper lane.
+            float f[6];
+            pm_offset_features(A.spos, i, A.force, A.n, A.box, A.inv_fscale, f);
+            uint16_t *c3 = act + swz<16>(r32, 3);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) c3[3 + q] = to_bf16(f[q]);
+            *reinterpret_cast<uint4 *>(act + swz<16>(r32, 4)) = make_uint4(pack_bf16(f[5], 0.f), 0u, 0u, 0u);
+#pragma unroll
+            for (int q = 5; q < 8; ++q) *reinterpret_cast<uint4 *>(act + swz<16>(r32, q)) = make_uint4(0u, 0u, 0u, 0u);
+        }
+        float z[4];
+        fused_layer<4, 4, true>(lds + kLdsW0, lbias, act, z, r32, half);
+        fused_layer<4, 8, true>(lds + kLdsW1, lbias + 128, act, z, r32, half);
+        fused_layer<4, 8, true>(lds + kLdsW2, lbias + 256, act, z, r32, half);
+        fused_layer<1, 8, false>(lds + kLdsW3, lbias + 384, act, z, r32, half);
+        if (half || !live) continue;
+        if (A.zout) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) A.zout[4 * t + r] = z[r];
+            continue;
+        }
+        // Lower lanes hold outputs 0-3 of their particle: the gate of k_apply with the calibrated
+        // multiplier kappa of the error estimate.
+        const float sx = sexp(z[0]) * A.fscale, sy = sexp(z[1]) * A.fscale, sz = sexp(z[2]) * A.fscale;
+        const float ymax = fmaxf(fabsf(z[0]), fmaxf(fabsf(z[1]), fabsf(z[2])));
+        const float sigma = sqrtf(3.f) * __expf(0.5f * fminf(z[3], 80.f) + ymax) * A.fscale;
+        if (A.pred) {
+            A.pred[4 * t] = sx;
+            A.pred[4 * t + 1] = sy;
+            A.pred[4 * t + 2] = sz;
+            A.pred[4 * t + 3] = sigma;
+            continue;
+        }
+        const size_t o = A.idx[i];
+        const float ax = A.acc[3 * o] + sx, ay = A.acc[3 * o + 1] + sy, az = A.acc[3 * o + 2] + sz;
+        const float amag = sqrtf(ax * ax + ay * ay + az * az);
+        const bool ok = isfinite(amag) && isfinite(sigma) && A.kappa * sigma <= A.tol * amag;
+        if (ok) {
+            A.acc[3 * o] = ax;
+            A.acc[3 * o + 1] = ay;
+            A.acc[3 * o + 2] = az;
+        }
+        A.reject[i] = ok ? 0u : 1u;
+    }
+}
+
+constexpr size_t kFusedLds = (size_t)kLdsActEnd * sizeof(uint16_t) + 416 * sizeof(float);
+static_assert(kFusedLds <= 160 * 1024, "fused inference LDS");
+
+// Calibration pairs of the listed sorted particles, relative to the predicted total force
+// |a| = |PM + F_pred| (the gate's normalisation): est[t] = sigma / |a|, err[t] = |F_pred - F_walk| / |a|.
+__global__ void k_calib_pairs(const float *__restrict__ pred, const uint32_t *__restrict__ sel, size_t count,
+                              const uint32_t *__restrict__ order, const float *__restrict__ acc_sr,
+                              const float *__restrict__ acc_pm, float *__restrict__ est, float *__restrict__ err) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const size_t o = order[sel[t]];
+    float d2 = 0.f, a2 = 0.f;
+    for (int c = 0; c < 3; ++c) {
+        const float d = pred[4 * t + c] - acc_sr[3 * o + c];
+        const float a = acc_pm[3 * o + c] + pred[4 * t + c];
+        d2 += d * d;
+        a2 += a * a;
+    }
+    const float amag = fmaxf(sqrtf(a2), 1e-30f);
+    est[t] = pred[4 * t + 3] / amag;
+    err[t] = sqrtf(d2) / amag;
 }
 
 // ------------------------------------------------------------------------------------ host
@@ -498,6 +755,129 @@ void mlp_gather_batch(Ctx &c, const uint16_t *Xall, const float *Tall, const uin
     hipLaunchKernelGGL(k_gather_batch, dim3(blocks(rows * 64, 256)), dim3(256), 0, c.stream, Xall, Tall, perm, b0,
                        count, (int)rows, L.X, L.T, L.Wrow);
     TPM_HIP(hipGetLastError());
+}
+
+static void launch_fused(Ctx &c, FusedArgs &a) {
+    if (a.count == 0) return;
+    Mlp &L = c.mlp;
+    static bool lds_set = false;
+    if (!lds_set) {
+        TPM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_fused),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFusedLds));
+        lds_set = true;
+    }
+    for (int l = 0; l < 4; ++l) {
+        a.wt[l] = L.wt16[l];
+        a.bias[l] = L.params + w_off(l) + (size_t)kMlpOut[l] * kMlpIn[l];
+    }
+    const size_t tiles = (a.count + 31) / 32;
+    const unsigned grid = (unsigned)std::min<size_t>((tiles + kFusedWaves - 1) / kFusedWaves,
+                                                     (size_t)std::max(1, c.num_cu));
+    hipLaunchKernelGGL(k_mlp_fused, dim3(grid), dim3(64 * kFusedWaves), kFusedLds, c.stream, a);
+    TPM_HIP(hipGetLastError());
+}
+
+// Fused inference over sorted particles sel[0..count) (or [first, first + count) when sel is null):
+// with pred == null the gate (acc and reject), else the calibration outputs (4 floats per particle
+// into pred: the predicted short-range force and the estimate sigma of its error).
+void mlp_infer_fused(Ctx &c, const uint32_t *sel, size_t first, size_t count, float fscale, float tol, float kappa,
+                     float *d_acc, uint32_t *reject, float *pred) {
+    FusedArgs a{};
+    a.spos = c.spos;
+    a.idx = c.idx;
+    a.delta = c.delta;
+    a.force = c.force;
+    a.n = c.n;
+    a.box = c.p.box_size;
+    a.sel = sel;
+    a.first = first;
+    a.count = count;
+    a.fscale = fscale;
+    a.inv_fscale = 1.f / fscale;
+    a.tol = tol;
+    a.kappa = kappa;
+    a.acc = d_acc;
+    a.reject = reject;
+    a.pred = pred;
+    launch_fused(c, a);
+}
+
+// The network on given BF16 feature rows (rows x 64): the four raw outputs per row into zout.
+void mlp_predict_rows(Ctx &c, const uint16_t *rows, size_t count, float *zout) {
+    FusedArgs a{};
+    a.rows = rows;
+    a.zout = zout;
+    a.count = count;
+    a.fscale = 1.f;
+    a.inv_fscale = 1.f;
+    launch_fused(c, a);
+}
+
+// Calibration of the gate on the listed (held-out) sorted particles: keeps their (estimate, error)
+// pairs, relative to |a|, sorted by estimate, for mlp_kappa.
+void mlp_calibrate(Ctx &c, const uint32_t *d_sel, size_t count, float fscale, const float *acc_sr,
+                   const float *acc_pm) {
+    Mlp &L = c.mlp;
+    L.cal_est.clear();
+    L.cal_err.clear();
+    L.kappa_cache.clear();
+    if (count == 0) return;
+    float *buf = nullptr;
+    TPM_HIP(hipMalloc(&buf, count * 6 * sizeof(float)));
+    std::vector<float> est(count), err(count);
+    try {
+        mlp_infer_fused(c, d_sel, 0, count, fscale, 0.f, 0.f, nullptr, nullptr, buf);
+        hipLaunchKernelGGL(k_calib_pairs, dim3(blocks(count, 256)), dim3(256), 0, c.stream, buf, d_sel, count, c.idx,
+                           acc_sr, acc_pm, buf + 4 * count, buf + 5 * count);
+        TPM_HIP(hipGetLastError());
+        TPM_HIP(hipMemcpyAsync(est.data(), buf + 4 * count, count * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipMemcpyAsync(err.data(), buf + 5 * count, count * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+    } catch (...) {
+        (void)hipFree(buf);
+        throw;
+    }
+    (void)hipFree(buf);
+    std::vector<size_t> ord(count);
+    for (size_t t = 0; t < count; ++t) ord[t] = t;
+    std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return est[x] < est[y]; });
+    L.cal_est.resize(count);
+    L.cal_err.resize(count);
+    for (size_t t = 0; t < count; ++t) {
+        L.cal_est[t] = std::isfinite(est[ord[t]]) ? est[ord[t]] : INFINITY;
+        L.cal_err[t] = std::isfinite(err[ord[t]]) ? err[ord[t]] : INFINITY;
+    }
+}
+
+// The gate's multiplier for tolerance tol: the gate accepts a particle when kappa sigma <= tol |a|,
+// i.e. when its relative estimate is at most tau = tol / kappa. tau is the largest calibration
+// estimate such that, among the calibration particles with estimates up to tau (at least 100 of
+// them), 99% have an actual error within tol; kappa = +inf (nothing accepted) when there is none.
+// Before any calibration the fixed multiplier 3 sqrt(3) applies.
+double mlp_kappa(Ctx &c, double tol) {
+    Mlp &L = c.mlp;
+    if (L.cal_est.empty()) return L.kappa;
+    if (!(tol > 0)) return INFINITY;
+    auto hit = L.kappa_cache.find(tol);
+    if (hit != L.kappa_cache.end()) return hit->second;
+    const size_t m = L.cal_est.size();
+    // Candidate prefix sizes on a geometric grid; the largest prefix that passes wins.
+    double best_tau = -1.0;
+    std::vector<float> e;
+    for (double f = 1.0; f >= 1e-4; f *= 0.9) {
+        const size_t k = (size_t)std::ceil(f * (double)m);
+        if (k < 100) break;
+        e.assign(L.cal_err.begin(), L.cal_err.begin() + k);
+        const size_t q = std::min(k - 1, (size_t)std::ceil(0.99 * (double)k) - 1);
+        std::nth_element(e.begin(), e.begin() + q, e.end());
+        if (e[q] <= tol && std::isfinite(L.cal_est[k - 1])) {
+            best_tau = k == m ? INFINITY : L.cal_est[k - 1];  // every calibration particle qualifies
+            break;
+        }
+    }
+    const double kappa = best_tau > 0 ? tol / best_tau : INFINITY;  // 0 when all qualify: accept all
+    L.kappa_cache[tol] = kappa;
+    return kappa;
 }
 
 void mlp_apply(Ctx &c, size_t count, size_t first, float fscale, float tol, float *d_acc, uint32_t *reject) {

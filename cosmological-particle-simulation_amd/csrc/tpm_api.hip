@@ -706,12 +706,10 @@ int tpm_mlp_predict(tpm_ctx *ctx, const float *x, size_t rows, float *out) {
         try {
             TPM_HIP(hipMemcpyAsync(dx, x, rows * 33 * sizeof(float), hipMemcpyHostToDevice, c.stream));
             tpm::mlp_pack_rows(c, dx, nullptr, rows, rp);
-            tpm::mlp_forward(c, rp, false);
+            // The inference kernel's network (the same arithmetic as the training forward pass).
+            tpm::mlp_predict_rows(c, c.mlp.X, rows, c.mlp.Zout);
             TPM_HIP(hipStreamSynchronize(c.stream));
-            std::vector<float> z(rp * 32);
-            TPM_HIP(hipMemcpy(z.data(), c.mlp.Zout, rp * 32 * sizeof(float), hipMemcpyDeviceToHost));
-            for (size_t r = 0; r < rows; ++r)
-                for (int k = 0; k < 4; ++k) out[4 * r + k] = z[32 * r + k];
+            TPM_HIP(hipMemcpy(out, c.mlp.Zout, rows * 4 * sizeof(float), hipMemcpyDeviceToHost));
         } catch (...) {
             (void)hipFree(dx);
             throw;
@@ -802,6 +800,15 @@ int tpm_mlp_train_dev(tpm_ctx *ctx, const double *d_pos, size_t n, double a, siz
                 }
                 if (loss_history) loss_history[e] = last;
             }
+            // Calibration of the gate on up to 65,536 particles midway between consecutive training samples
+            // (held out when the samples are at least two apart; the training samples themselves otherwise).
+            const size_t gap = n / ns;
+            const size_t mcal = std::min<size_t>(65536, ns);
+            std::vector<uint32_t> cal(mcal);
+            for (size_t t = 0; t < mcal; ++t)
+                cal[t] = (uint32_t)std::min<uint64_t>(n - 1, sel[t * ns / mcal] + (gap >= 2 ? gap / 2 : 0));
+            TPM_HIP(hipMemcpyAsync(d_sel, cal.data(), mcal * sizeof(uint32_t), hipMemcpyHostToDevice, c.stream));
+            tpm::mlp_calibrate(c, d_sel, mcal, (float)fscale, c.acc_sr, c.acc);
             TPM_HIP(hipStreamSynchronize(c.stream));
         } catch (...) {
             cleanup();
@@ -819,8 +826,23 @@ __global__ void k_compact(const uint32_t *__restrict__ flag, const uint32_t *__r
     if (i < n && flag[i]) out[pos[i]] = (uint32_t)i;
 }
 
+// walked[idx[i]] = reject[i]: the fallback mask in caller order.
+__global__ void k_walked_mask(const uint32_t *__restrict__ reject, const uint32_t *__restrict__ order, size_t n,
+                              uint32_t *__restrict__ walked) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) walked[order[i]] = reject[i];
+}
+
+int tpm_mlp_error_scale(tpm_ctx *ctx, double tol, double *kappa) {
+    return guarded(ctx, [&](Ctx &c) {
+        TPM_REQUIRE(kappa != nullptr, "null output");
+        *kappa = tpm::mlp_kappa(c, tol);
+        return TPM_OK;
+    });
+}
+
 int tpm_accelerations_learned_dev(tpm_ctx *ctx, const double *d_pos, size_t n, double a, double tol, float *d_acc,
-                                  uint64_t *n_fallback) {
+                                  uint64_t *n_fallback, uint32_t *d_walked) {
     return guarded(ctx, [&](Ctx &c) {
         TPM_REQUIRE(d_pos && d_acc && n > 0 && n <= c.max_n, "bad particle set");
         TPM_REQUIRE((c.p.flags & TPM_FLAG_TREEPM) != 0, "the learned term replaces the TreePM short-range force");
@@ -834,13 +856,13 @@ int tpm_accelerations_learned_dev(tpm_ctx *ctx, const double *d_pos, size_t n, d
         tm.mark();
         pm_part(c, d_pos, n, a, d_acc);
         tm.mark();
-        const size_t chunk = 1u << 20;
-        tpm::mlp_prepare(c, chunk);
-        for (size_t first = 0; first < n; first += chunk) {
-            const size_t cnt = std::min(chunk, n - first);
-            tpm::mlp_features(c, nullptr, first, cnt, c.mlp.X, (float)(1.0 / fscale));
-            tpm::mlp_forward(c, round32(cnt), false);
-            tpm::mlp_apply(c, cnt, first, (float)fscale, (float)tol, d_acc, c.reject);
+        const double kappa = tpm::mlp_kappa(c, tol);
+        tpm::mlp_infer_fused(c, nullptr, 0, n, (float)fscale, (float)tol, std::isfinite(kappa) ? (float)kappa : INFINITY,
+                             d_acc, c.reject, nullptr);
+        if (d_walked) {
+            hipLaunchKernelGGL(k_walked_mask, dim3(tpm::blocks(n, 256)), dim3(256), 0, c.stream, c.reject, c.idx, n,
+                               d_walked);
+            TPM_HIP(hipGetLastError());
         }
         tm.mark();
         // Members = rejected sorted indices (exclusive scan + scatter).
@@ -854,8 +876,10 @@ int tpm_accelerations_learned_dev(tpm_ctx *ctx, const double *d_pos, size_t n, d
         const size_t nrej = (size_t)last[0] + last[1];
         if (n_fallback) *n_fallback = nrej;
         if (nrej > 0) {
+            tpm::zero_fill(c, c.counter, 8 * sizeof(unsigned long long));
             tpm::build_tree(c, n, !(c.p.flags & TPM_FLAG_REF_WALK));
             tpm::walk_short_range(c, n, k, d_acc, c.members, nrej);
+            tpm::fold_status(c, true);
         }
         tm.mark();
         if (c.timing) {

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -90,6 +91,9 @@ struct Mlp {
     uint16_t *w16[4] = {nullptr, nullptr, nullptr, nullptr};   // BF16 W [in x out]
     int step = 0;
     bool trained = false;
+    double kappa = 5.196152422706632;  // error-estimate multiplier of the gate until calibrated (3 sqrt(3))
+    std::vector<float> cal_est, cal_err;  // calibration pairs (sigma / |a|, error / |a|), sorted by estimate
+    std::map<double, double> kappa_cache;  // tol -> multiplier
     size_t cap = 0;  // rows of the work buffers
     uint16_t *X = nullptr, *Xt = nullptr, *Y[3] = {nullptr, nullptr, nullptr}, *Yt[3] = {nullptr, nullptr, nullptr};
     float *Z[3] = {nullptr, nullptr, nullptr}, *C = nullptr, *Zout = nullptr, *dY = nullptr, *T = nullptr,
@@ -322,6 +326,12 @@ void mlp_targets(Ctx &c, const uint32_t *d_sel, size_t count, const float *acc_s
 void mlp_gather_batch(Ctx &c, const uint16_t *Xall, const float *Tall, const uint32_t *perm, size_t b0,
                       size_t count, size_t rows);
 void mlp_apply(Ctx &c, size_t count, size_t first, float fscale, float tol, float *d_acc, uint32_t *reject);
+void mlp_infer_fused(Ctx &c, const uint32_t *sel, size_t first, size_t count, float fscale, float tol, float kappa,
+                     float *d_acc, uint32_t *reject, float *pred);
+void mlp_predict_rows(Ctx &c, const uint16_t *rows, size_t count, float *zout);
+void mlp_calibrate(Ctx &c, const uint32_t *d_sel, size_t count, float fscale, const float *acc_sr,
+                   const float *acc_pm);
+double mlp_kappa(Ctx &c, double tol);
 
 // Plans and tables (fft_kernels.hip).
 void make_plan(Ctx &c);

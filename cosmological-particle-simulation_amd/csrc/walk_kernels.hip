@@ -771,12 +771,11 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     if (n == 0) return;
     if (members && count == 0) return;
     if (c.p.flags & TPM_FLAG_REF_WALK) {
-        TPM_REQUIRE(members == nullptr, "the reference walk does not take a member list");
         RefWalkArgs<float> w{};
         w.spos = c.spos;
         w.idx = c.idx;
-        w.targets = nullptr;
-        w.m = n;
+        w.targets = members;  // the per-particle walk of each listed particle (or of all)
+        w.m = members ? count : n;
         w.t = c.tree.view();
         w.box = (float)c.p.box_size;
         w.r_split = (float)c.p.r_split;
@@ -787,7 +786,7 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
         w.out_f64 = nullptr;
         w.acc = d_acc;
         w.counter = c.counter;
-        hipLaunchKernelGGL(k_ref_walk<float>, dim3(blocks(n, 128)), dim3(128), 0, c.stream, w);
+        hipLaunchKernelGGL(k_ref_walk<float>, dim3(blocks(w.m, 128)), dim3(128), 0, c.stream, w);
         TPM_HIP(hipGetLastError());
         return;
     }

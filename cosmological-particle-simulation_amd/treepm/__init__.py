@@ -54,7 +54,7 @@ EXPORTS = [
     "tpm_snapshot_dev", "tpm_run_settings_default", "tpm_run_settings_sanitized", "tpm_headless_run", "tpm_downsample_grid_dev", "tpm_autotune", "tpm_set_tuning", "tpm_ensemble_create", "tpm_ensemble_destroy", "tpm_ensemble_last_error",
     "tpm_ensemble_ics", "tpm_ensemble_autotune", "tpm_ensemble_run", "tpm_ensemble_get",
     "tpm_snapshot_write", "tpm_snapshot_read", "tpm_deposit_dev_f64_noalloc", "tpm_get_device_status",
-    "tpm_morton_order", "tpm_ensemble_graph_nodes",
+    "tpm_morton_order", "tpm_ensemble_graph_nodes", "tpm_mlp_error_scale",
 ]
 
 
@@ -244,7 +244,8 @@ def load():
         "tpm_mlp_gradients": (ctypes.c_int, [P, P, P, S, P, ctypes.POINTER(ctypes.c_double)]),
         "tpm_mlp_predict": (ctypes.c_int, [P, P, S, P]),
         "tpm_mlp_train_dev": (ctypes.c_int, [P, P, S, D, S, ctypes.c_int, S, F, ctypes.c_uint64, P]),
-        "tpm_accelerations_learned_dev": (ctypes.c_int, [P, P, S, D, D, P, ctypes.POINTER(ctypes.c_uint64)]),
+        "tpm_accelerations_learned_dev": (ctypes.c_int, [P, P, S, D, D, P, ctypes.POINTER(ctypes.c_uint64), P]),
+        "tpm_mlp_error_scale": (ctypes.c_int, [P, D, ctypes.POINTER(ctypes.c_double)]),
         "tpm_growth": (ctypes.c_int, [ctypes.POINTER(Cosmology), D, P, P, P]),
         "tpm_power_spectrum": (ctypes.c_int, [ctypes.POINTER(Cosmology), D, D, D, D, P, P]),
         "tpm_power_sigma": (ctypes.c_int, [ctypes.POINTER(Cosmology), D, D, D, D, P]),
@@ -523,11 +524,19 @@ class Solver:
                                                int(epochs), int(batch), float(lr), int(seed), _ptr(hist)))
         return hist[:int(epochs)]
 
-    def accelerations_learned_dev(self, d_pos, n, a, tol, d_acc):
+    def accelerations_learned_dev(self, d_pos, n, a, tol, d_acc, d_walked=0):
+        """Learned TreePM force; returns the number of walked particles (d_walked: optional device mask)."""
         nf = ctypes.c_uint64()
         self._check(self.lib.tpm_accelerations_learned_dev(self.h, ctypes.c_void_p(d_pos), int(n), float(a),
-                                                           float(tol), ctypes.c_void_p(d_acc), ctypes.byref(nf)))
+                                                           float(tol), ctypes.c_void_p(d_acc), ctypes.byref(nf),
+                                                           ctypes.c_void_p(d_walked or None)))
         return int(nf.value)
+
+    def mlp_error_scale(self, tol):
+        """The gate's calibrated error-estimate multiplier kappa for tolerance tol (inf: accepts nothing)."""
+        v = ctypes.c_double()
+        self._check(self.lib.tpm_mlp_error_scale(self.h, float(tol), ctypes.byref(v)))
+        return v.value
 
     def sync(self):
         self._check(self.lib.tpm_sync(self.h))

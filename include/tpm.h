@@ -228,8 +228,9 @@ int tpm_nfw_halos_dev(tpm_ctx *ctx, const tpm_nfw_spec *spec, size_t n, double *
 /*
  * Learned short-range correction (no reference counterpart; the exact walk is its fallback and
  * judge). A 64 -> 128 -> 128 -> 128 -> 4 SiLU MLP with BF16 weights and activations on MFMA (FP32
- * accumulation) predicts the short-range force from 33 features per particle (log(1 + delta) on
- * the 3x3x3 cells around it, the PM force, its offset from the cell centre in cells) and a
+ * accumulation) predicts the short-range force from 33 features per particle (log(1 + delta) of its
+ * cell and, relative to that, of the 26 cells around it; the PM force; its offset from the cell
+ * centre in cells) and a
  * log-variance used as the error estimate. Forces are in the compressed units slog(F / (K/h^2)),
  * slog(x) = sign(x) log(1 + |x|), with K the coupling and h the cell size.
  * Parameters are FP32 masters (tpm_mlp_param_count() floats, layout: per layer Wt [out x in] then
@@ -246,18 +247,32 @@ int tpm_mlp_gradients(tpm_ctx *ctx, const float *x, const float *t, size_t rows,
 int tpm_mlp_predict(tpm_ctx *ctx, const float *x, size_t rows, float *out);
 /*
  * Trains on n_samples particles of d_pos (device, FP64), with the walk's short-range force as the
- * target, for `epochs` passes of Adam (batch rows, learning rate lr). loss_history (epochs
- * entries, may be NULL) receives each epoch's last-batch loss.
+ * target, for `epochs` passes of Adam (batch rows, learning rate lr), then calibrates the gate on up
+ * to 65,536 particles midway between training samples (their predicted and actual errors, see
+ * tpm_mlp_error_scale). loss_history (epochs entries, may be NULL) receives each epoch's last-batch
+ * loss.
  */
 int tpm_mlp_train_dev(tpm_ctx *ctx, const double *d_pos, size_t n, double a, size_t n_samples, int epochs,
                       size_t batch, float lr, uint64_t seed, double *loss_history);
 /*
- * TreePM force with the learned short-range term: PM force + the MLP's prediction wherever its
- * estimated RMS error is within tol * |a|; every other particle is walked exactly. n_fallback
- * (may be NULL) receives the number of walked particles.
+ * TreePM force with the learned short-range term: PM force + the MLP's prediction wherever its error
+ * estimate, kappa * sigma, is within tol * |a| (sigma: the network's predicted RMS error propagated to
+ * force units; kappa: tpm_mlp_error_scale for this tol, from the calibration of tpm_mlp_train_dev);
+ * every other particle is walked exactly (the context's walk: the group walk, or
+ * with TPM_FLAG_REF_WALK the per-particle walk, whose result for a particle does not depend on which
+ * others are walked). Features, network and gate run as one kernel. n_fallback (may be NULL)
+ * receives the number of walked particles; d_walked (device, n words in caller order, may be NULL)
+ * receives 1 for each walked particle and 0 for each predicted one.
  */
 int tpm_accelerations_learned_dev(tpm_ctx *ctx, const double *d_pos, size_t n, double a, double tol, float *d_acc,
-                                  uint64_t *n_fallback);
+                                  uint64_t *n_fallback, uint32_t *d_walked);
+/*
+ * The gate's error-estimate multiplier kappa for tolerance tol. After training: tol / tau, with tau the
+ * largest relative estimate sigma / |a| of the calibration particles such that 99% of those with
+ * estimates up to tau (at least 100) have actual errors within tol * |a|; +inf when none qualify (the
+ * gate then accepts nothing). Before any training: 3 sqrt(3).
+ */
+int tpm_mlp_error_scale(tpm_ctx *ctx, double tol, double *kappa);
 
 /* Background cosmology for the KDK step factors (src/cosmology.rs:9-34, flat: Or + Om + OL = 1). */
 typedef struct tpm_cosmology {
