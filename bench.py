@@ -209,8 +209,7 @@ def learned_vs_plain(args):
     hist = s.mlp_train_dev(pos.ptr, npart, a, args.learned_samples, args.learned_epochs, batch=8192, lr=1e-3, seed=1)
     s.sync()
     train_s = time.perf_counter() - t0
-    kappa = s.mlp_error_scale()
-    log(f"[bench] learned: trained in {train_s:.1f} s, loss {hist[0]:.3f} -> {hist[-1]:.3f}, kappa {kappa:.3f}")
+    log(f"[bench] learned: trained in {train_s:.1f} s, loss {hist[0]:.3f} -> {hist[-1]:.3f}")
     # the plain curve, log-log, for the equal-error comparison
     pe = np.log([p["rms"] for p in plain])
     pt = np.log([p["ms"] for p in plain])
@@ -226,7 +225,9 @@ def learned_vs_plain(args):
         ms, nf = timed(lambda: s.accelerations_learned_dev(pos.ptr, npart, a, tol, acc.ptr))
         e = error()
         at = plain_ms_at(e["rms"])
-        learned.append(dict(tol=(tol if tol < 1e29 else "inf"), ms=ms, fallback_fraction=nf / npart,
+        kappa = s.mlp_error_scale(tol)
+        learned.append(dict(tol=(tol if tol < 1e29 else "inf"), kappa=(kappa if np.isfinite(kappa) else None), ms=ms,
+                            fallback_fraction=nf / npart,
                             plain_ms_at_equal_rms=at, speedup_at_equal_rms=(at / ms if at else None), **e))
         log(f"[bench] learned: tol {tol:g}: {ms:.1f} ms, fallback {nf / npart:.3f}, rms {e['rms']:.3e}, "
             f"equal-error speed-up {learned[-1]['speedup_at_equal_rms']}")
@@ -236,7 +237,7 @@ def learned_vs_plain(args):
     best = max((p["speedup_at_equal_rms"] or 0.0) for p in learned)
     return {"workload": f"treepm_nfw20000_{n}cubed_mesh{n}_z0", "train_samples": int(min(args.learned_samples, npart)),
             "epochs": args.learned_epochs, "train_s": train_s, "loss_history": [float(v) for v in hist],
-            "kappa": kappa, "tuning": tuning, "plain": plain, "learned": learned,
+            "tuning": tuning, "plain": plain, "learned": learned,
             "best_speedup_at_equal_rms": best,
             "judge": "FP64 TreePM (theta 0.2) on a seeded 16,384-particle subsample",
             "note": "speedup_at_equal_rms: the plain curve's time at the learned point's RMS error (log-log "
