@@ -202,7 +202,14 @@ def learned_vs_plain(args):
         ms, _ = timed(lambda: s.accelerations_dev(pos.ptr, npart, a, acc.ptr))
         plain.append(dict(theta=th, ms=ms, **error()))
         log(f"[bench] learned: plain theta {th}: {ms:.1f} ms, rms {plain[-1]['rms']:.3e}")
-    s.set_params(theta=args.theta)
+    # The cheaper splits of the main workload's legs on this input: the walk they leave and their error.
+    split_probe = []
+    for rs in [float(v) for v in args.split_legs.split(",") if v.strip()]:
+        s.set_params(theta=args.theta, r_split=rs * BOX / n, r_cut=4.5 * rs * BOX / n)
+        ms, _ = timed(lambda: s.accelerations_dev(pos.ptr, npart, a, acc.ptr))
+        split_probe.append(dict(r_split_cells=rs, theta=args.theta, ms=ms, **error()))
+        log(f"[bench] learned: r_split {rs} cells: {ms:.1f} ms, rms {split_probe[-1]['rms']:.3e}")
+    s.set_params(theta=args.theta, r_split=split["r_split"], r_cut=split["r_cut"])
     log(f"[bench] learned: training on {args.learned_samples} walk samples, {args.learned_epochs} epochs")
     s.mlp_init(1)
     t0 = time.perf_counter()
@@ -237,7 +244,7 @@ def learned_vs_plain(args):
     best = max((p["speedup_at_equal_rms"] or 0.0) for p in learned)
     return {"workload": f"treepm_nfw20000_{n}cubed_mesh{n}_z0", "train_samples": int(min(args.learned_samples, npart)),
             "epochs": args.learned_epochs, "train_s": train_s, "loss_history": [float(v) for v in hist],
-            "tuning": tuning, "plain": plain, "learned": learned,
+            "tuning": tuning, "plain": plain, "split_legs": split_probe, "learned": learned,
             "best_speedup_at_equal_rms": best,
             "judge": "FP64 TreePM (theta 0.2) on a seeded 16,384-particle subsample",
             "note": "speedup_at_equal_rms: the plain curve's time at the learned point's RMS error (log-log "
@@ -308,6 +315,8 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=256, help="particles per side of the CPU baseline's sample")
     ap.add_argument("--theta-legs", type=str, default="0.5",
                     help="comma-separated extra opening angles timed and judged on the same input ('' = none)")
+    ap.add_argument("--split-legs", type=str, default="0.4,0.5,0.625",
+                    help="comma-separated smaller split scales r_s (mesh cells) timed and judged on the same input")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-size", type=int, default=512)
     ap.add_argument("--learned-samples", type=int, default=20_000_000)
@@ -452,6 +461,7 @@ def main():
         log("[bench] FP64 judge ...")
         ref = solver.judge(host_pos, idx, a, theta=0.2)
         d = host_acc[idx].astype(np.float64) - ref
+        del host_acc
         rel = np.sqrt((d ** 2).sum(1)) / np.sqrt((ref ** 2).sum(1))
         result["force_error"] = {"rms": float(np.sqrt(np.mean(rel ** 2))), "p99": float(np.percentile(rel, 99)),
                                  "median": float(np.median(rel)), "sample": int(m),
@@ -459,7 +469,30 @@ def main():
         log(f"[bench] force error rms {result['force_error']['rms']:.3e} p99 {result['force_error']['p99']:.3e}")
 
         # The same input at other opening angles (the reference's default is 0.5, src/runner.rs:59),
-        # with the knobs tuned above and with the default knobs: time per evaluation and error.
+        # with the knobs tuned above and with the default knobs: time per evaluation and error against
+        # the same judge forces (the judge does not depend on the opening angle).
+        ref0 = ref
+
+        def timed_leg():
+            solver.set_timing(True)
+            solver.accelerations_dev(pos.ptr, npart, a, acc.ptr)
+            solver.sync()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                solver.accelerations_dev(pos.ptr, npart, a, acc.ptr)
+            solver.sync()
+            ms = 1000.0 * (time.perf_counter() - t0) / 2
+            st = solver.stats()
+            solver.set_timing(False)
+            got = acc.numpy().reshape(-1, 3)[idx].astype(np.float64)
+            rel = np.sqrt(((got - ref0) ** 2).sum(1)) / np.sqrt((ref0 ** 2).sum(1))
+            return {"ms_per_eval": ms, "evals_per_s": npart / (ms / 1000.0), "walk_kernel_ms": st["ms_walk_kernel"],
+                    "rms": float(np.sqrt(np.mean(rel ** 2))), "p99": float(np.percentile(rel, 99))}
+
+        def tuned_knobs():
+            solver.set_tuning(tuning["fft_cap"], tuning["group_level_shift"], fft_path=tuning["fft_path"],
+                              tree_depth=tuning["tree_depth"], small_node_direct=tuning["small_node_direct"])
+
         legs = []
         for th in [float(v) for v in args.theta_legs.split(",") if v.strip()]:
             for knobs in ("tuned", "default"):
@@ -468,31 +501,29 @@ def main():
                 if knobs == "default":
                     solver.set_tuning(16, 0, fft_path=0, tree_depth=21, small_node_direct=0)
                 else:
-                    solver.set_tuning(tuning["fft_cap"], tuning["group_level_shift"], fft_path=tuning["fft_path"],
-                                      tree_depth=tuning["tree_depth"], small_node_direct=tuning["small_node_direct"])
+                    tuned_knobs()
                 solver.set_params(theta=th)
-                solver.set_timing(True)
-                solver.accelerations_dev(pos.ptr, npart, a, acc.ptr)
-                solver.sync()
-                t0 = time.perf_counter()
-                for _ in range(2):
-                    solver.accelerations_dev(pos.ptr, npart, a, acc.ptr)
-                solver.sync()
-                ms = 1000.0 * (time.perf_counter() - t0) / 2
-                st = solver.stats()
-                solver.set_timing(False)
-                host_acc = acc.numpy().reshape(-1, 3)
-                ref = solver.judge(host_pos, idx, a, theta=0.2)
-                d = host_acc[idx].astype(np.float64) - ref
-                rel = np.sqrt((d ** 2).sum(1)) / np.sqrt((ref ** 2).sum(1))
-                legs.append({"theta": th, "knobs": knobs, "ms_per_eval": ms, "evals_per_s": npart / (ms / 1000.0),
-                             "walk_kernel_ms": st["ms_walk_kernel"], "rms": float(np.sqrt(np.mean(rel ** 2))),
-                             "p99": float(np.percentile(rel, 99))})
-                log(f"[bench] theta {th} ({knobs} knobs): {ms:.1f} ms, rms {legs[-1]['rms']:.3e} "
+                legs.append(dict(theta=th, knobs=knobs, **timed_leg()))
+                log(f"[bench] theta {th} ({knobs} knobs): {legs[-1]['ms_per_eval']:.1f} ms, rms {legs[-1]['rms']:.3e} "
                     f"p99 {legs[-1]['p99']:.3e}")
         if legs:
             result["theta_legs"] = legs
-        del host_pos, host_acc
+        # A cheaper force split on the same input (smaller r_s, r_cut = 4.5 r_s: a shorter walk and more
+        # of the force from the mesh), timed and judged against the configured split's FP64 forces: how
+        # much of the walk a smaller cutoff saves and what error it leaves behind.
+        split_legs = []
+        if tuning is not None:
+            tuned_knobs()
+        for rs in [float(v) for v in args.split_legs.split(",") if v.strip()]:
+            h = BOX / n
+            solver.set_params(theta=args.theta, r_split=rs * h, r_cut=4.5 * rs * h)
+            split_legs.append(dict(r_split_cells=rs, theta=args.theta, **timed_leg()))
+            log(f"[bench] r_split {rs} cells: {split_legs[-1]['ms_per_eval']:.1f} ms, rms {split_legs[-1]['rms']:.3e}")
+        solver.set_params(theta=args.theta, r_split=split["r_split"], r_cut=split["r_cut"])
+        if split_legs:
+            result["split_legs"] = {"note": "error against the FP64 judge of the configured split (r_s = 1.25 cells)",
+                                    "legs": split_legs}
+        del host_pos
 
     solver.close()
     pos.free()
