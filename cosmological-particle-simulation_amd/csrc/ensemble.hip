@@ -279,6 +279,8 @@ int tpm_ensemble_create(const tpm_ensemble_params *params, int nboxes, const int
             p.theta = params->theta;
             const int rc = tpm_create(&p, bx.n, &bx.ctx);
             if (rc != TPM_OK) throw tpm::Error{rc, std::string("box context: ") + tpm_last_error(nullptr)};
+            // No box reports interaction counts: its walks skip the per-pair counter (same forces).
+            tpm_ctx_internal(bx.ctx).count_pairs = false;
             TPM_HIP(hipMalloc(&bx.pos, 3 * bx.n * sizeof(double)));
             TPM_HIP(hipMalloc(&bx.mom, 3 * bx.n * sizeof(double)));
             TPM_HIP(hipMalloc(&bx.pos0, 3 * bx.n * sizeof(double)));

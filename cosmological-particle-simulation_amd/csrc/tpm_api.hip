@@ -123,7 +123,7 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     TPM_HIP(hipMalloc(&c.gstart, (mn + 1) * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.gscratch, 2 * mn * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.ngroups_dev, 16));
-    TPM_HIP(hipMalloc(&c.walk_ctr, 16 * sizeof(uint32_t)));
+    TPM_HIP(hipMalloc(&c.walk_ctr, 8 * 1024 * sizeof(uint32_t)));
     c.dep_level = 0;
     while (((c.n + (1 << c.dep_level) - 1) >> c.dep_level) > 8) ++c.dep_level;
     {

@@ -174,7 +174,7 @@ struct Ctx {
     int tree_tab_level = 0;
     size_t tree_tab_n = 0;
     uint32_t *ngroups_dev = nullptr;  // walk group count (device)
-    uint32_t *walk_ctr = nullptr;     // walk group counters, one per XCD range (16 words)
+    uint32_t *walk_ctr = nullptr;     // walk group counters, one per XCD range, 1024 words apart (8192 words)
     uint32_t *dep_start = nullptr;    // per deposit cube: first sorted index, item count, first item (3 (8^dep_level + 1))
     int dep_level = 0;                // Morton level of the deposit cubes (at most 8 cells a side)
 };

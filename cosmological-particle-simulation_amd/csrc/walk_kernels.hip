@@ -621,6 +621,10 @@ __device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w
 // counter; when the range is exhausted it moves on to the next one, so every wavefront reaches
 // the exit after visiting each range once and the load stays balanced.
 constexpr int kSlots = 8;
+// The ranges' counters sit 4 KiB apart, in lines that different memory channels serve, so their
+// atomics proceed in parallel: every wavefront passes every counter once on its way out, which on
+// a small box costs more than the walk when the counters share one line.
+constexpr int kSlotStride = 1024;
 
 // Six wavefronts per SIMD: the register budget is held to 80 VGPRs. CNT: count the interactions
 // (one vector instruction per pair); without it the interaction total is not reported.
@@ -644,7 +648,7 @@ __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_e
         const uint32_t r0 = (uint32_t)(((uint64_t)ng * slot) / kSlots);
         const uint32_t r1 = (uint32_t)(((uint64_t)ng * (slot + 1)) / kSlots);
         uint32_t g = 0;
-        if (lane == 0) g = atomicAdd(w.next_group + slot, 1u);
+        if (lane == 0) g = atomicAdd(w.next_group + slot * kSlotStride, 1u);
         g = __builtin_amdgcn_readfirstlane(g);
         if (g >= r1 - r0) {
             slot = (slot + 1) % kSlots;
@@ -653,10 +657,11 @@ __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_e
         }
         walk_group<CNT>(sh, bs.stab, w, r0 + g, tot);
     }
+    // Statistics; a wavefront that walked no group adds nothing (on a small box most of them).
     if (lane == 0 && w.counter) {
-        if (CNT) atomicAdd(w.counter, tot.inter);
-        atomicAdd(w.counter + 2, tot.listed);
-        atomicAdd(w.counter + 3, tot.kept);
+        if (CNT && tot.inter) atomicAdd(w.counter, tot.inter);
+        if (tot.listed) atomicAdd(w.counter + 2, tot.listed);
+        if (tot.kept) atomicAdd(w.counter + 3, tot.kept);
     }
 }
 
@@ -829,15 +834,17 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     w.members = members;
     w.ngroups = c.ngroups_dev;
     w.next_group = c.walk_ctr;
-    zero_fill(c, w.next_group, kSlots * sizeof(uint32_t));
+    zero_fill(c, w.next_group, kSlots * kSlotStride * sizeof(uint32_t));
     if (c.walk_blocks == 0) {
         int per_cu = 0;
         TPM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_walk<true>, 64 * kWaves, 0));
         c.walk_blocks = (unsigned)std::max(1, per_cu) * (unsigned)c.num_cu;
     }
-    // Persistent grid; workgroups without a group left exit at once. Never more workgroups than
-    // a bound on the group count needs.
-    const size_t bound = ((members ? count : n) + kWaves - 1) / kWaves;
+    // Persistent grid; workgroups without a group left exit at once. No more workgroups than one
+    // wavefront per 32 particles (groups hold 64 at most and about 50 at mean density; with fewer
+    // wavefronts than groups each wavefront takes several): idle wavefronts cost their exit
+    // protocol, which on a small box outweighs the walk.
+    const size_t bound = ((members ? count : n) + kWaves * 32 - 1) / (kWaves * 32);
     const unsigned nblk = (unsigned)std::max<size_t>(1, std::min<size_t>(c.walk_blocks, bound));
     if (c.timing) {
         if (!c.walk_ev[0]) TPM_HIP(hipEventCreate(&c.walk_ev[0]));
