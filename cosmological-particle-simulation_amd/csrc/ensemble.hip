@@ -354,8 +354,9 @@ int tpm_ensemble_autotune(tpm_ensemble *e, double a, tpm_tuning *out) {
     });
 }
 
-// Enqueues initial force + `steps` KDK steps of every box: eagerly (use_graph = 0, each box on its
-// own stream) or by one launch of the captured graph.
+// Enqueues initial force + `steps` KDK steps of every box: eagerly (use_graph = 0; the walks of the
+// boxes of one side batched into one launch per evaluation) or by one launch of the captured graph
+// (per-box chains).
 int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start, double a_end, int steps,
                      int use_graph, double *ms) {
     return eguard(e, [&](tpm_ensemble &E) {
@@ -370,6 +371,11 @@ int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start
         std::vector<double> a(steps + 1);
         a[0] = a_start;
         for (int i = 0; i < steps; ++i) a[i + 1] = a_start * std::pow(a_end / a_start, (double)(i + 1) / steps);
+        // Per-box chains (the captured form): the whole run of each box on its own stream, each
+        // evaluation with its own walk. A captured run built from the batched form below (the walks
+        // of one side joined across the boxes' streams at every evaluation) crashed the host process
+        // inside the HIP runtime at 96 boxes and 21 evaluations (no GPU fault; small ensembles ran
+        // correctly), so graph mode keeps the per-box chains.
         auto enqueue = [&](tpm::Box &bx) {
             Ctx &c = tpm_ctx_internal(bx.ctx);
             tpm::evaluate(c, bx.pos, bx.n, a[0], false, bx.acc, nullptr);
@@ -381,21 +387,78 @@ int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start
                 tpm::kick(c, bx.mom, bx.acc, bx.n, f.k2);
             }
         };
+        // Boxes of one side share every scalar of the walk (split, softening and coupling in cells, the
+        // tuned knobs): each evaluation walks them in one launch, on the stream of the side's first
+        // box, once every box of the side has built its tree and groups. Everything else of an
+        // evaluation runs on the box's own stream; the boxes' forces are those of their own walks.
+        std::vector<std::vector<size_t>> sides;
+        for (size_t b = 0; b < E.boxes.size(); ++b) {
+            size_t q = 0;
+            while (q < sides.size() && E.boxes[sides[q][0]].side != E.boxes[b].side) ++q;
+            if (q == sides.size()) sides.emplace_back();
+            sides[q].push_back(b);
+        }
+        std::vector<hipEvent_t> events;
+        auto new_event = [&]() {
+            hipEvent_t ev;
+            TPM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            events.push_back(ev);
+            return ev;
+        };
+        auto stream_of = [&](size_t b) { return tpm_ctx_internal(E.boxes[b].ctx).stream; };
+        // Enqueues the whole run after `start` (an event on the master stream); returns with the
+        // master stream waiting for every box.
+        auto enqueue_all = [&](hipEvent_t start) {
+            for (size_t b = 0; b < E.boxes.size(); ++b) TPM_HIP(hipStreamWaitEvent(stream_of(b), start, 0));
+            std::vector<tpm::WalkArgs> wa(E.boxes.size());
+            std::vector<hipEvent_t> pre(E.boxes.size());
+            for (int i = 0; i <= steps; ++i) {
+                tpm::KdkFactors f{};
+                if (i > 0) f = tpm::kdk_factors(*cosmo, a[i - 1], a[i]);
+                for (size_t b = 0; b < E.boxes.size(); ++b) {
+                    tpm::Box &bx = E.boxes[b];
+                    Ctx &c = tpm_ctx_internal(bx.ctx);
+                    if (i > 0) {
+                        tpm::kick(c, bx.mom, bx.acc, bx.n, f.k1);
+                        tpm::drift(c, bx.pos, bx.mom, bx.n, f.d);
+                    }
+                    wa[b] = tpm::evaluate_pre_walk(c, bx.pos, bx.n, a[i], bx.acc);
+                    pre[b] = new_event();
+                    TPM_HIP(hipEventRecord(pre[b], c.stream));
+                }
+                for (const auto &sd : sides) {
+                    Ctx &lead = tpm_ctx_internal(E.boxes[sd[0]].ctx);
+                    std::vector<tpm::WalkArgs> batch;
+                    size_t total = 0;
+                    for (size_t b : sd) {
+                        if (b != sd[0]) TPM_HIP(hipStreamWaitEvent(lead.stream, pre[b], 0));
+                        batch.push_back(wa[b]);
+                        total += E.boxes[b].n;
+                    }
+                    tpm::walk_launch_batch(lead, batch, total);
+                    hipEvent_t walked = new_event();
+                    TPM_HIP(hipEventRecord(walked, lead.stream));
+                    for (size_t b : sd) {
+                        tpm::Box &bx = E.boxes[b];
+                        Ctx &c = tpm_ctx_internal(bx.ctx);
+                        if (b != sd[0]) TPM_HIP(hipStreamWaitEvent(c.stream, walked, 0));
+                        tpm::evaluate_post_walk(c, bx.acc, bx.n);
+                        if (i > 0) tpm::kick(c, bx.mom, bx.acc, bx.n, f.k2);
+                    }
+                }
+            }
+            for (size_t b = 0; b < E.boxes.size(); ++b) {
+                hipEvent_t done = new_event();
+                TPM_HIP(hipEventRecord(done, stream_of(b)));
+                TPM_HIP(hipStreamWaitEvent(E.master, done, 0));
+            }
+        };
         hipEvent_t t0, t1;
         TPM_HIP(hipEventCreate(&t0));
         TPM_HIP(hipEventCreate(&t1));
         if (!use_graph) {
             TPM_HIP(hipEventRecord(t0, E.master));
-            for (auto &bx : E.boxes) {
-                Ctx &c = tpm_ctx_internal(bx.ctx);
-                TPM_HIP(hipStreamWaitEvent(c.stream, t0, 0));
-                enqueue(bx);
-                hipEvent_t done;
-                TPM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
-                TPM_HIP(hipEventRecord(done, c.stream));
-                TPM_HIP(hipStreamWaitEvent(E.master, done, 0));
-                (void)hipEventDestroy(done);
-            }
+            enqueue_all(t0);
             TPM_HIP(hipEventRecord(t1, E.master));
         } else {
             if (!E.exec || E.g_a0 != a_start || E.g_a1 != a_end || E.g_steps != steps) {
@@ -415,24 +478,17 @@ int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start
                                            c.stream));
                     TPM_HIP(hipStreamSynchronize(c.stream));
                 }
-                hipEvent_t fork;
-                TPM_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+                hipEvent_t fork = new_event();
                 TPM_HIP(hipStreamBeginCapture(E.master, hipStreamCaptureModeRelaxed));
                 TPM_HIP(hipEventRecord(fork, E.master));
-                std::vector<hipEvent_t> joins;
-                for (auto &bx : E.boxes) {
-                    Ctx &c = tpm_ctx_internal(bx.ctx);
-                    TPM_HIP(hipStreamWaitEvent(c.stream, fork, 0));
-                    enqueue(bx);
-                    hipEvent_t done;
-                    TPM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
-                    TPM_HIP(hipEventRecord(done, c.stream));
+                for (size_t b = 0; b < E.boxes.size(); ++b) {
+                    TPM_HIP(hipStreamWaitEvent(stream_of(b), fork, 0));
+                    enqueue(E.boxes[b]);
+                    hipEvent_t done = new_event();
+                    TPM_HIP(hipEventRecord(done, stream_of(b)));
                     TPM_HIP(hipStreamWaitEvent(E.master, done, 0));
-                    joins.push_back(done);
                 }
                 TPM_HIP(hipStreamEndCapture(E.master, &E.graph));
-                for (auto ev : joins) (void)hipEventDestroy(ev);
-                (void)hipEventDestroy(fork);
                 // Every operation of a run must be a kernel (memset nodes have given wrong results on
                 // replays after the first on these machines): count the node types and refuse a
                 // graph that holds a memset.
@@ -461,6 +517,7 @@ int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start
             TPM_HIP(hipEventRecord(t1, E.master));
         }
         TPM_HIP(hipEventSynchronize(t1));
+        for (auto ev : events) (void)hipEventDestroy(ev);
         for (auto &bx : E.boxes) tpm::finish_checks(tpm_ctx_internal(bx.ctx));
         float t = 0.f;
         TPM_HIP(hipEventElapsedTime(&t, t0, t1));

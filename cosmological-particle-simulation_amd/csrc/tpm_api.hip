@@ -83,7 +83,7 @@ static void free_all(Ctx &c) {
     void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.radix_table, c.scan_part, c.spos,
                     c.counter, c.flag, c.status, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
                     c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.tree_scan,
-                    c.ngroups_dev, c.walk_ctr, c.dep_start, c.inv, c.judge_buf, c.gstart, c.gscratch, c.stab,
+                    c.ngroups_dev, c.walk_ctr, c.walk_goff, c.dep_start, c.inv, c.judge_buf, c.gstart, c.gscratch, c.stab,
                     c.acc_sr, c.members, c.reject, c.mlp.params, c.mlp.grads, c.mlp.m1, c.mlp.m2};
     for (int l = 0; l < 4; ++l) {
         if (c.mlp.wt16[l]) (void)hipFree(c.mlp.wt16[l]);
@@ -124,6 +124,7 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     TPM_HIP(hipMalloc(&c.gscratch, 2 * mn * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.ngroups_dev, 16));
     TPM_HIP(hipMalloc(&c.walk_ctr, 8 * 1024 * sizeof(uint32_t)));
+    TPM_HIP(hipMalloc(&c.walk_goff, 64 * sizeof(uint32_t)));
     c.dep_level = 0;
     while (((c.n + (1 << c.dep_level) - 1) >> c.dep_level) > 8) ++c.dep_level;
     {
@@ -248,6 +249,29 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
         c.stats.kept = cnt[3];
         if (cnt[1]) throw Error{TPM_E_DEVICE, "walk stack overflow"};
     }
+}
+
+// evaluate() for FP64 positions, split around the short-range walk's launch so that the walks of
+// several boxes can share one launch: the same kernels in the same order (sort, deposit, PM,
+// interpolation into d_acc, octree, walk groups), then the prepared walk's arguments.
+WalkArgs evaluate_pre_walk(Ctx &c, const double *d_pos, size_t n, double a, float *d_acc) {
+    TPM_REQUIRE(n > 0 && n <= c.max_n, "bad particle count for a batched evaluation");
+    TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
+    TPM_REQUIRE((c.p.flags & TPM_FLAG_TREEPM) && !(c.p.flags & TPM_FLAG_REF_WALK),
+                "a batched evaluation needs the TreePM split and the group walk");
+    zero_fill(c, c.counter, 8 * sizeof(unsigned long long));
+    sort_particles(c, d_pos, n);
+    deposit_fixed(c, n, 1.0, false, c.spos, true);
+    pm_from_grid(c, n, a, c.p.r_split, nullptr);
+    interp_force(c, n, false, d_acc);
+    build_tree(c, n, true);
+    return walk_prepare(c, n, coupling(c, a, n, false), d_acc);
+}
+
+// The end of evaluate() after the walk: the finiteness check when asked and the status fold.
+void evaluate_post_walk(Ctx &c, float *d_acc, size_t n) {
+    if (c.p.flags & TPM_FLAG_CHECK_FINITE) check_finite(c, d_acc, 3 * n);
+    fold_status(c, true);
 }
 
 void finish_checks(Ctx &c) {

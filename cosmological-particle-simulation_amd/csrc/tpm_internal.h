@@ -175,6 +175,7 @@ struct Ctx {
     size_t tree_tab_n = 0;
     uint32_t *ngroups_dev = nullptr;  // walk group count (device)
     uint32_t *walk_ctr = nullptr;     // walk group counters, one per XCD range, 1024 words apart (8192 words)
+    uint32_t *walk_goff = nullptr;    // first global group of each box of a batched walk (kWalkBatch + 1)
     uint32_t *dep_start = nullptr;    // per deposit cube: first sorted index, item count, first item (3 (8^dep_level + 1))
     int dep_level = 0;                // Morton level of the deposit cubes (at most 8 cells a side)
 };
@@ -228,6 +229,64 @@ void morton_sort_positions(Ctx &c, const double *d_pos, size_t n, int depth);
 // S(u) table of the group walk for the context's r_split and r_cut.
 void make_short_range_table(Ctx &c);
 
+// Arguments of the group walk (walk_kernels.hip).
+struct WalkArgs {
+    const double *spos;
+    const uint32_t *idx;
+    const uint32_t *gstart;
+    const uint32_t *members;  // sorted indices of the walked particles (null: all, in order)
+    size_t n;
+    TreeView t;
+    double box;
+    double inv_box;
+    double rcut;
+    double theta2;
+    int small_direct;  // nodes of at most kWalkLeaf particles are listed even where they could be accepted
+    // The FP32 pair arithmetic works in coordinates scaled by c = sqrt(kSTab / s_max), where the S
+    // table, uniform in s = r^2 + eps^2 on [0, s_max], has unit spacing: s' = c^2 s is the table
+    // coordinate itself. All FP32 lengths below are scaled; forces come out unscaled because the
+    // couplings carry c^2 (f' dx' = K c^2 S / (c^3 rho^3) c dx = K S dx / rho^3).
+    double scale;  // c
+    float boxf;
+    float invboxf;
+    float cut2;
+    float cut2s;  // r_cut^2 + eps^2 (the pair cutoff on s2 = r^2 + eps^2)
+    float eps2;
+    float rcutf;  // r_cut, scaled
+    float scalef;
+    float inv2rs;
+    float negk;  // -K c^2
+    float tabk;  // -K c^2
+    const float2 *stab;
+    uint32_t *next_group;
+    const uint32_t *ngroups;  // device word written by the group builder
+    float *acc;
+    unsigned long long *counter;
+};
+
+// One box of a batched walk: the per-box arrays of a prepared walk (the scalars are shared).
+struct WalkBox {
+    const double *spos;
+    const uint32_t *idx;
+    const uint32_t *gstart;
+    const uint32_t *ngroups;
+    TreeView t;
+    float *acc;
+    unsigned long long *counter;
+};
+// Boxes per batched launch: the table travels in the kernel arguments (at most 4 KiB).
+constexpr int kWalkBatch = 40;
+
+// The group walk up to its launch: walk groups, counters and the launch arguments (group walk
+// only; the reference walk has no batched form).
+WalkArgs walk_prepare(Ctx &c, size_t n, double k, float *d_acc);
+// Launches a prepared walk of c's particles on c.stream.
+void walk_launch(Ctx &c, const WalkArgs &w, size_t n);
+// One launch (per kWalkBatch boxes) walking the prepared walks of boxes that share every scalar
+// of their arguments (same split, softening, coupling, opening angle and small-node rule), on
+// lead.stream; each box's forces are those of its own walk_launch.
+void walk_launch_batch(Ctx &lead, const std::vector<WalkArgs> &boxes, size_t total_particles);
+
 // Short-range force of every particle added into acc[idx[i]] (group walk or reference walk).
 // With members != null only the `count` listed sorted particles are walked (all n are sources).
 void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *members = nullptr,
@@ -248,6 +307,10 @@ void downsample_grid(Ctx &c, const float *delta, float *out);
 // Force evaluation on device-resident FP64 positions (tpm_api.hip); no host synchronisation
 // unless timing is on.
 void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float *d_acc, float *d_delta);
+// evaluate() (FP64 positions, group walk) split around the walk's launch, for batched walks:
+// everything before the launch, returning its arguments; and what follows the walk.
+WalkArgs evaluate_pre_walk(Ctx &c, const double *d_pos, size_t n, double a, float *d_acc);
+void evaluate_post_walk(Ctx &c, float *d_acc, size_t n);
 // Enqueues the fold of this evaluation's failure flags (walk stack overflow, octree capacity when
 // the tree was built, non-finite accelerations when checked) into the sticky c.status word.
 void fold_status(Ctx &c, bool tree_built);

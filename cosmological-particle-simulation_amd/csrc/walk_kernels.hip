@@ -36,39 +36,6 @@ namespace tpm {
 
 constexpr int kSTab = 1024;  // intervals of the S table on [0, s_max]
 
-struct WalkArgs {
-    const double *spos;
-    const uint32_t *idx;
-    const uint32_t *gstart;
-    const uint32_t *members;  // sorted indices of the walked particles (null: all, in order)
-    size_t n;
-    TreeView t;
-    double box;
-    double inv_box;
-    double rcut;
-    double theta2;
-    int small_direct;  // nodes of at most kWalkLeaf particles are listed even where they could be accepted
-    // The FP32 pair arithmetic works in coordinates scaled by c = sqrt(kSTab / s_max), where the S
-    // table, uniform in s = r^2 + eps^2 on [0, s_max], has unit spacing: s' = c^2 s is the table
-    // coordinate itself. All FP32 lengths below are scaled; forces come out unscaled because the
-    // couplings carry c^2 (f' dx' = K c^2 S / (c^3 rho^3) c dx = K S dx / rho^3).
-    double scale;  // c
-    float boxf;
-    float invboxf;
-    float cut2;
-    float cut2s;  // r_cut^2 + eps^2 (the pair cutoff on s2 = r^2 + eps^2)
-    float eps2;
-    float rcutf;  // r_cut, scaled
-    float scalef;
-    float inv2rs;
-    float negk;  // -K c^2
-    float tabk;  // -K c^2
-    const float2 *stab;
-    uint32_t *next_group;
-    const uint32_t *ngroups;  // device word written by the group builder
-    float *acc;
-    unsigned long long *counter;
-};
 
 constexpr float kErfP = 0.3275911f;
 constexpr float kErfA1 = 0.254829592f;
@@ -431,7 +398,7 @@ __device__ __forceinline__ void eval_nodes(WaveShared &sh, int nn, const WalkArg
 }
 
 template <bool WRAP, bool SELF, bool CNT>
-__device__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkArgs &w, double gx, double gy,
+__device__ __forceinline__ void group_walk_body(WaveShared &sh, const float2 *stab, const WalkArgs &w, double gx, double gy,
                                 double gz, double hx, double hy, double hz, uint32_t self, float xi, float yi,
                                 float zi, bool active, float &ax, float &ay, float &az, uint32_t &ninter,
                                 uint32_t &listed, uint32_t &kept, uint32_t &tag) {
@@ -555,7 +522,7 @@ struct WaveTotals {
 };
 
 template <bool CNT>
-__device__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w, uint32_t g, WaveTotals &tot) {
+__device__ __forceinline__ void walk_group(WaveShared &sh, const float2 *stab, const WalkArgs &w, uint32_t g, WaveTotals &tot) {
     const int lane = threadIdx.x & 63;
     const size_t g0 = w.gstart[g];
     const size_t g1 = w.gstart[g + 1];
@@ -658,6 +625,80 @@ __global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_e
         walk_group<CNT>(sh, bs.stab, w, r0 + g, tot);
     }
     // Statistics; a wavefront that walked no group adds nothing (on a small box most of them).
+    if (lane == 0 && w.counter) {
+        if (CNT && tot.inter) atomicAdd(w.counter, tot.inter);
+        if (tot.listed) atomicAdd(w.counter + 2, tot.listed);
+        if (tot.kept) atomicAdd(w.counter + 3, tot.kept);
+    }
+}
+
+// The boxes of a batched walk, passed by value in the kernel arguments; goff[b] is the first global
+// group of box b (goff[nbox]: all groups), written by k_group_offsets before the walk.
+struct WalkBoxArgs {
+    WalkBox b[kWalkBatch];
+    uint32_t nbox;
+    const uint32_t *goff;
+};
+
+__global__ void k_group_offsets(WalkBoxArgs bx, uint32_t *__restrict__ goff) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t s = 0;
+    goff[0] = 0;
+    for (uint32_t b = 0; b < bx.nbox; ++b) {
+        s += *bx.b[b].ngroups;
+        goff[b + 1] = s;
+    }
+}
+
+// k_group_walk over the groups of several boxes: the global group list is the boxes' lists one
+// after another, split into the same XCD ranges; a group's box is found by a binary search of
+// goff and the box's arrays replace those of the shared arguments. Each group is walked exactly
+// as k_group_walk walks it in its own box.
+template <bool CNT>
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(6, 8)))
+k_group_walk_batch(WalkArgs w, WalkBoxArgs bx) {
+    __shared__ BlockShared bs;
+    for (int i = threadIdx.x; i <= kSTab; i += blockDim.x) {
+        const float2 t = w.stab[i];
+        bs.stab[i] = make_float2(w.tabk * t.x, w.tabk * t.y);
+    }
+    __syncthreads();
+    WaveShared &sh = bs.wave[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    sh.mark[lane] = 0xFFFFFFFFu;  // never a tag
+    wave_sync();
+    const uint32_t ng = bx.goff[bx.nbox];
+    int slot = (int)(blockIdx.x % kSlots);
+    WaveTotals tot;
+    for (int visited = 0; visited < kSlots;) {
+        const uint32_t r0 = (uint32_t)(((uint64_t)ng * slot) / kSlots);
+        const uint32_t r1 = (uint32_t)(((uint64_t)ng * (slot + 1)) / kSlots);
+        uint32_t g = 0;
+        if (lane == 0) g = atomicAdd(w.next_group + slot * kSlotStride, 1u);
+        g = __builtin_amdgcn_readfirstlane(g);
+        if (g >= r1 - r0) {
+            slot = (slot + 1) % kSlots;
+            ++visited;
+            continue;
+        }
+        const uint32_t gg = r0 + g;
+        uint32_t lo = 0, hi = bx.nbox - 1;  // last box with goff <= gg
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (bx.goff[mid] <= gg)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        WalkArgs wb = w;
+        wb.spos = bx.b[lo].spos;
+        wb.idx = bx.b[lo].idx;
+        wb.gstart = bx.b[lo].gstart;
+        wb.t = bx.b[lo].t;
+        wb.acc = bx.b[lo].acc;
+        wb.counter = bx.b[lo].counter;
+        walk_group<CNT>(sh, bs.stab, wb, gg - bx.goff[lo], tot);
+    }
     if (lane == 0 && w.counter) {
         if (CNT && tot.inter) atomicAdd(w.counter, tot.inter);
         if (tot.listed) atomicAdd(w.counter + 2, tot.listed);
@@ -772,6 +813,8 @@ __global__ void k_gather_keys(const uint64_t *__restrict__ keys, const uint32_t 
     if (t < count) out[t] = keys[members[t]];
 }
 
+WalkArgs walk_prepare(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *members, size_t count);
+
 void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *members, size_t count) {
     if (n == 0) return;
     if (members && count == 0) return;
@@ -795,6 +838,11 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
         TPM_HIP(hipGetLastError());
         return;
     }
+    const WalkArgs w = walk_prepare(c, n, k, d_acc, members, count);
+    walk_launch(c, w, members ? count : n);
+}
+
+WalkArgs walk_prepare(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *members, size_t count) {
     WalkArgs w{};
     w.spos = c.spos;
     w.idx = c.idx;
@@ -835,6 +883,12 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     w.ngroups = c.ngroups_dev;
     w.next_group = c.walk_ctr;
     zero_fill(c, w.next_group, kSlots * kSlotStride * sizeof(uint32_t));
+    return w;
+}
+
+WalkArgs walk_prepare(Ctx &c, size_t n, double k, float *d_acc) { return walk_prepare(c, n, k, d_acc, nullptr, 0); }
+
+static unsigned walk_grid(Ctx &c, size_t walked) {
     if (c.walk_blocks == 0) {
         int per_cu = 0;
         TPM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_group_walk<true>, 64 * kWaves, 0));
@@ -844,8 +898,12 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
     // wavefront per 32 particles (groups hold 64 at most and about 50 at mean density; with fewer
     // wavefronts than groups each wavefront takes several): idle wavefronts cost their exit
     // protocol, which on a small box outweighs the walk.
-    const size_t bound = ((members ? count : n) + kWaves * 32 - 1) / (kWaves * 32);
-    const unsigned nblk = (unsigned)std::max<size_t>(1, std::min<size_t>(c.walk_blocks, bound));
+    const size_t bound = (walked + kWaves * 32 - 1) / (kWaves * 32);
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(c.walk_blocks, bound));
+}
+
+void walk_launch(Ctx &c, const WalkArgs &w, size_t walked) {
+    const unsigned nblk = walk_grid(c, walked);
     if (c.timing) {
         if (!c.walk_ev[0]) TPM_HIP(hipEventCreate(&c.walk_ev[0]));
         if (!c.walk_ev[1]) TPM_HIP(hipEventCreate(&c.walk_ev[1]));
@@ -857,6 +915,37 @@ void walk_short_range(Ctx &c, size_t n, double k, float *d_acc, const uint32_t *
         hipLaunchKernelGGL(k_group_walk<false>, dim3(nblk), dim3(64 * kWaves), 0, c.stream, w);
     TPM_HIP(hipGetLastError());
     if (c.timing) TPM_HIP(hipEventRecord(c.walk_ev[1], c.stream));
+}
+
+void walk_launch_batch(Ctx &lead, const std::vector<WalkArgs> &boxes, size_t total_particles) {
+    static_assert(sizeof(WalkArgs) + sizeof(WalkBoxArgs) <= 4096, "walk batch arguments exceed 4 KiB");
+    TPM_REQUIRE(!boxes.empty(), "empty walk batch");
+    for (size_t b0 = 0; b0 < boxes.size(); b0 += kWalkBatch) {
+        const size_t nb = std::min<size_t>(kWalkBatch, boxes.size() - b0);
+        WalkBoxArgs bx{};
+        bx.nbox = (uint32_t)nb;
+        bx.goff = lead.walk_goff;
+        size_t walked = 0;
+        for (size_t q = 0; q < nb; ++q) {
+            const WalkArgs &a = boxes[b0 + q];
+            TPM_REQUIRE(a.members == nullptr, "a batched walk takes whole boxes");
+            bx.b[q] = WalkBox{a.spos, a.idx, a.gstart, a.ngroups, a.t, a.acc, a.counter};
+            walked += a.n;
+        }
+        (void)total_particles;
+        WalkArgs w = boxes[b0];  // the shared scalars, the S table and the lead's counters
+        w.next_group = lead.walk_ctr;
+        w.counter = lead.counter;
+        if (b0 > 0) zero_fill(lead, lead.walk_ctr, kSlots * kSlotStride * sizeof(uint32_t));
+        hipLaunchKernelGGL(k_group_offsets, dim3(1), dim3(64), 0, lead.stream, bx, lead.walk_goff);
+        TPM_HIP(hipGetLastError());
+        const unsigned nblk = walk_grid(lead, walked);
+        if (lead.count_pairs)
+            hipLaunchKernelGGL(k_group_walk_batch<true>, dim3(nblk), dim3(64 * kWaves), 0, lead.stream, w, bx);
+        else
+            hipLaunchKernelGGL(k_group_walk_batch<false>, dim3(nblk), dim3(64 * kWaves), 0, lead.stream, w, bx);
+        TPM_HIP(hipGetLastError());
+    }
 }
 
 // S(u) = erfc(u) + (2/sqrt(pi)) u exp(-u^2) (src/tree.rs:54-57), u = r / (2 r_s), tabulated in FP64
