@@ -222,6 +222,99 @@ tpm_tuning autotune(Ctx &c, const double *d_pos, size_t n, double a, float *d_ac
     return t;
 }
 
+// Walk knobs of the boxes of one side (group cube level, small-node rule), chosen again by the time
+// of the batched walk that an eager ensemble run makes (one launch for all boxes of the side): the
+// per-box tuner times a box alone, whose walk cannot fill the GPU. Each candidate is timed twice;
+// the accuracy guard is the per-box tuner's (the first box's force error against the FP64 judge at
+// most 1.1 times that of the default knobs).
+static void tune_side_batched(tpm_ensemble &E, const std::vector<size_t> &sd, double a) {
+    Box &L = E.boxes[sd[0]];
+    Ctx &lead = tpm_ctx_internal(L.ctx);
+    for (size_t b : sd) {
+        Box &bx = E.boxes[b];
+        Ctx &c = tpm_ctx_internal(bx.ctx);
+        TPM_HIP(hipMemcpyAsync(bx.pos, bx.pos0, 3 * bx.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        TPM_HIP(hipStreamSynchronize(c.stream));
+    }
+    TuneRef ref;
+    tune_reference(lead, L.pos, L.n, a, ref);
+    const double err_cap = 1.1 * L.tuning.err_default + 1e-7;
+    std::vector<hipEvent_t> ev(sd.size() + 1, nullptr);
+    for (auto &e : ev) TPM_HIP(hipEventCreate(&e));
+    auto set_knobs = [&](int shift, int small) {
+        for (size_t b : sd) {
+            Ctx &c = tpm_ctx_internal(E.boxes[b].ctx);
+            c.group_level_shift = shift;
+            c.small_direct = small;
+        }
+    };
+    // Batched walk time of the side (min of two runs) and the first box's force error.
+    auto run = [&](double *err) {
+        float best = 1e30f;
+        for (int r = 0; r < 2; ++r) {
+            std::vector<WalkArgs> batch;
+            size_t total = 0;
+            for (size_t q = 0; q < sd.size(); ++q) {
+                Box &bx = E.boxes[sd[q]];
+                Ctx &c = tpm_ctx_internal(bx.ctx);
+                batch.push_back(evaluate_pre_walk(c, bx.pos, bx.n, a, bx.acc));
+                total += bx.n;
+                if (q > 0) {
+                    TPM_HIP(hipEventRecord(ev[q], c.stream));
+                    TPM_HIP(hipStreamWaitEvent(lead.stream, ev[q], 0));
+                }
+            }
+            TPM_HIP(hipEventRecord(ev[0], lead.stream));
+            walk_launch_batch(lead, batch, total);
+            TPM_HIP(hipEventRecord(ev[sd.size()], lead.stream));
+            for (size_t q = 0; q < sd.size(); ++q) {
+                Box &bx = E.boxes[sd[q]];
+                Ctx &c = tpm_ctx_internal(bx.ctx);
+                if (q > 0) TPM_HIP(hipStreamWaitEvent(c.stream, ev[sd.size()], 0));
+                evaluate_post_walk(c, bx.acc, bx.n);
+            }
+            for (size_t b : sd) TPM_HIP(hipStreamSynchronize(tpm_ctx_internal(E.boxes[b].ctx).stream));
+            float t = 0.f;
+            TPM_HIP(hipEventElapsedTime(&t, ev[0], ev[sd.size()]));
+            best = std::min(best, t);
+        }
+        *err = tune_error(lead, L.acc, ref);
+        return (double)best;
+    };
+    int best_shift = L.tuning.group_level_shift, best_small = L.tuning.small_node_direct;
+    double best_t = 1e30, best_e = L.tuning.err_tuned;
+    try {
+        set_knobs(best_shift, best_small);
+        double e0;
+        best_t = run(&e0);
+        best_e = e0;
+        for (int shift : {-1, 0, 1, 2})
+            for (int small : {0, 1}) {
+                if (shift == L.tuning.group_level_shift && small == L.tuning.small_node_direct) continue;
+                set_knobs(shift, small);
+                double e;
+                const double t = run(&e);
+                if (t < best_t && e <= err_cap) {
+                    best_t = t;
+                    best_e = e;
+                    best_shift = shift;
+                    best_small = small;
+                }
+            }
+    } catch (...) {
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    set_knobs(best_shift, best_small);
+    for (size_t b : sd) {
+        Box &bx = E.boxes[b];
+        bx.tuning.group_level_shift = best_shift;
+        bx.tuning.small_node_direct = best_small;
+        bx.tuning.err_tuned = best_e;
+    }
+}
+
 }  // namespace tpm
 
 using tpm::Ctx;
@@ -343,8 +436,19 @@ int tpm_ensemble_autotune(tpm_ensemble *e, double a, tpm_tuning *out) {
                 c.small_direct = bx.tuning.small_node_direct;
                 c.group_level_shift = bx.tuning.group_level_shift;
             }
-            if (out) out[b] = bx.tuning;
         }
+        // The walk knobs again, for the batched walk of each side with more than one box.
+        std::vector<std::vector<size_t>> sides;
+        for (size_t b = 0; b < E.boxes.size(); ++b) {
+            size_t q = 0;
+            while (q < sides.size() && E.boxes[sides[q][0]].side != E.boxes[b].side) ++q;
+            if (q == sides.size()) sides.emplace_back();
+            sides[q].push_back(b);
+        }
+        for (const auto &sd : sides)
+            if (sd.size() > 1) tpm::tune_side_batched(E, sd, a);
+        if (out)
+            for (size_t b = 0; b < E.boxes.size(); ++b) out[b] = E.boxes[b].tuning;
         // Tuned knobs change the recorded launches: the graph is rebuilt on the next run.
         if (E.exec) (void)hipGraphExecDestroy(E.exec);
         if (E.graph) (void)hipGraphDestroy(E.graph);
