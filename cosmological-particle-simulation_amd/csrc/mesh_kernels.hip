@@ -196,35 +196,93 @@ __global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__
     __syncthreads();
     const double scale = mass * 0x1p32;
     const unsigned long long total = (unsigned long long)llrint(scale);
-    for (uint32_t i = p0 + tid; i < p1; i += 256) {
-        int64_t base[3];
-        T frac[3];
+    if (starts[k + 1] - starts[k] > kDepItem) {
+        // A dense cube (more than one item: at least four particles per cell on average). Each
+        // thread takes a contiguous run of the sorted order, whose consecutive particles in dense
+        // regions share their CIC base cell, and sums their quanta per corner in registers, adding
+        // them to the tile only when the base cell changes: the LDS atomics of particles crowded
+        // into few cells, which serialise on equal addresses, become one per run. Integer sums:
+        // the grid is bit-identical to adding every quantum on its own.
+        const uint32_t run = (p1 - p0 + 255) / 256;
+        const uint32_t q0 = p0 + tid * run, q1 = min(p1, q0 + run);
+        const int s1 = dim[2], s0 = dim[1] * dim[2];
+        const int corner[8] = {0, 1, s1, s1 + 1, s0, s0 + 1, s0 + s1, s0 + s1 + 1};
+        int cur = -1;
+        unsigned long long sum[8];
+        for (uint32_t i = q0; i < q1; ++i) {
+            int64_t base[3];
+            T frac[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) cic_axis<T>((T)pos[3 * (size_t)i + a], h, base[a], frac[a]);
-        bool inside = true;
-        int off[3];
+            for (int a = 0; a < 3; ++a) cic_axis<T>((T)pos[3 * (size_t)i + a], h, base[a], frac[a]);
+            bool inside = true;
+            int off[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int64_t o = base[a] - lo[a];
-            inside = inside && o >= 0 && o <= dim[a] - 2;
-            off[a] = (int)o;
+            for (int a = 0; a < 3; ++a) {
+                const int64_t o = base[a] - lo[a];
+                inside = inside && o >= 0 && o <= dim[a] - 2;
+                off[a] = (int)o;
+            }
+            Stencil<T> st;
+            if (!inside) cic_corners<T>(base, frac, n, st);
+            const int t0 = inside ? (off[0] * dim[1] + off[1]) * dim[2] + off[2] : -1;
+            if (inside && t0 != cur) {
+                if (cur >= 0)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) atomicAdd(&tile[cur + corner[c]], sum[c]);
+                cur = t0;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) sum[c] = 0ull;
+            }
+            unsigned long long used = 0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int dx = (c >> 2) & 1, dy = (c >> 1) & 1, dz = c & 1;
+                const T wx = dx == 0 ? (T)1 - frac[0] : frac[0];
+                const T wy = dy == 0 ? (T)1 - frac[1] : frac[1];
+                const T wz = dz == 0 ? (T)1 - frac[2] : frac[2];
+                const T w = (wx * wy) * wz;
+                const unsigned long long q = c < 7 ? (unsigned long long)llrint((double)w * scale) : total - used;
+                used += q;
+                if (inside)
+                    sum[c] += q;
+                else
+                    atomicAdd(grid + st.idx[c], q);
+            }
         }
-        Stencil<T> st;
-        if (!inside) cic_corners<T>(base, frac, n, st);
-        unsigned long long used = 0;
+        if (cur >= 0)
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int dx = (c >> 2) & 1, dy = (c >> 1) & 1, dz = c & 1;
-            const T wx = dx == 0 ? (T)1 - frac[0] : frac[0];
-            const T wy = dy == 0 ? (T)1 - frac[1] : frac[1];
-            const T wz = dz == 0 ? (T)1 - frac[2] : frac[2];
-            const T w = (wx * wy) * wz;
-            const unsigned long long q = c < 7 ? (unsigned long long)llrint((double)w * scale) : total - used;
-            used += q;
-            if (inside)
-                atomicAdd(&tile[((off[0] + dx) * dim[1] + (off[1] + dy)) * dim[2] + (off[2] + dz)], q);
-            else
-                atomicAdd(grid + st.idx[c], q);
+            for (int c = 0; c < 8; ++c) atomicAdd(&tile[cur + corner[c]], sum[c]);
+    } else {
+        for (uint32_t i = p0 + tid; i < p1; i += 256) {
+            int64_t base[3];
+            T frac[3];
+    #pragma unroll
+            for (int a = 0; a < 3; ++a) cic_axis<T>((T)pos[3 * (size_t)i + a], h, base[a], frac[a]);
+            bool inside = true;
+            int off[3];
+    #pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int64_t o = base[a] - lo[a];
+                inside = inside && o >= 0 && o <= dim[a] - 2;
+                off[a] = (int)o;
+            }
+            Stencil<T> st;
+            if (!inside) cic_corners<T>(base, frac, n, st);
+            unsigned long long used = 0;
+    #pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int dx = (c >> 2) & 1, dy = (c >> 1) & 1, dz = c & 1;
+                const T wx = dx == 0 ? (T)1 - frac[0] : frac[0];
+                const T wy = dy == 0 ? (T)1 - frac[1] : frac[1];
+                const T wz = dz == 0 ? (T)1 - frac[2] : frac[2];
+                const T w = (wx * wy) * wz;
+                const unsigned long long q = c < 7 ? (unsigned long long)llrint((double)w * scale) : total - used;
+                used += q;
+                if (inside)
+                    atomicAdd(&tile[((off[0] + dx) * dim[1] + (off[1] + dy)) * dim[2] + (off[2] + dz)], q);
+                else
+                    atomicAdd(grid + st.idx[c], q);
+            }
         }
     }
     __syncthreads();

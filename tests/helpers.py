@@ -85,3 +85,30 @@ def mode_amplitude(delta, n, box_size):
     x = (np.arange(n) + 0.5) * box_size / n
     c = np.cos(2 * np.pi / box_size * x)
     return 2.0 * float((delta.reshape(n, n, n) * c[:, None, None]).sum()) / n ** 3
+
+
+def fixed_point_delta32(pos32, n, box_size):
+    """Density contrast of libtpm's fixed-point deposit with FP32 CIC weights and unit masses:
+    s = x / h - 0.5 and the weights (wx wy) wz in FP32 as the reference computes them
+    (src/mesh.rs:29-69), each corner's quantum round(w 2^32) with the last corner taking the
+    remainder of round(2^32), summed exactly as integers; delta = c / mean - 1 in FP32."""
+    pos32 = np.asarray(pos32, np.float32).reshape(-1, 3)
+    h = np.float32(box_size) / np.float32(n)
+    s = pos32 / h - np.float32(0.5)
+    i0 = np.floor(s)
+    base = i0.astype(np.int64)
+    frac = (s - i0).astype(np.float32)
+    total = np.int64(np.rint(2.0 ** 32))
+    grid = np.zeros(n ** 3, np.int64)
+    used = np.zeros(len(pos32), np.int64)
+    one = np.float32(1)
+    for c in range(8):
+        d = ((c >> 2) & 1, (c >> 1) & 1, c & 1)
+        w = [frac[:, a] if d[a] else one - frac[:, a] for a in range(3)]
+        wt = (w[0] * w[1]) * w[2]
+        q = np.rint(wt.astype(np.float64) * 2.0 ** 32).astype(np.int64) if c < 7 else total - used
+        used += q
+        idx = [(base[:, a] + d[a]) % n for a in range(3)]
+        np.add.at(grid, (idx[0] * n + idx[1]) * n + idx[2], q)
+    mean32 = np.float32(len(pos32)) / np.float32(n ** 3)
+    return (grid.astype(np.float64) * 2.0 ** -32).astype(np.float32) / mean32 - np.float32(1)

@@ -13,7 +13,8 @@ import os
 import numpy as np
 import pytest
 
-from helpers import OMEGA_M, lattice, max_diff, max_norm, mesh_scattered, rel_errors, tree_scattered
+from helpers import (OMEGA_M, fixed_point_delta32, lattice, max_diff, max_norm, mesh_scattered, rel_errors,
+                     tree_scattered)
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -62,6 +63,34 @@ def test_deposit_bit_exact_and_order_independent(treepm):
     g3 = s.deposit(pos[rng.permutation(len(pos))])
     assert np.array_equal(g1.view(np.uint32), g2.view(np.uint32))
     assert np.array_equal(g1.view(np.uint32), g3.view(np.uint32))
+
+
+def test_dense_cube_deposit_in_evaluation(treepm, oracle):
+    # The evaluation's deposit over Morton cubes takes the run-aggregated path for a cube of more
+    # than one work item: 20,000 particles crowded into about one cell of a 32^3 mesh put far more
+    # than 2,048 particles into one 8-cell cube. Its density contrast must match the oracle's (FP32
+    # weights and FP64 weights), and the integer grid must not depend on the input order.
+    rng = np.random.default_rng(5)
+    box, n = 100.0, 32
+    pos = rng.random((60000, 3)) * box
+    pos[:20000] = 40.0 + rng.random((20000, 3)) * 3.0
+    pos32 = pos.astype(np.float32)
+    s = solver(treepm, n, box, max_particles=len(pos))
+    _, d32 = s.accelerations(pos32, 0.5, want_delta=True)
+    perm = rng.permutation(len(pos))
+    _, d32p = s.accelerations(pos32[perm], 0.5, want_delta=True)
+    assert np.array_equal(d32.view(np.uint32), d32p.view(np.uint32))
+    # The fixed-point sum is exact, so the result is known bit for bit: the reference's FP32
+    # weights, each particle's quanta summed as integers. (The FP32 oracle's serial FP32 sum over
+    # the ~4,000 particles of a clump cell is itself 3e-6 off, and FP32 against FP64 weights
+    # differ by 2.2e-6 of 1 + delta there.)
+    assert np.array_equal(d32.view(np.uint32), fixed_point_delta32(pos32, n, box).view(np.uint32))
+    ref32 = oracle.density_contrast(pos32, n, box)
+    assert np.max(np.abs(d32 - ref32) / np.maximum(np.abs(ref32 + 1), 1.0)) < 5e-6
+    _, d64 = s.accelerations(pos, 0.5, want_delta=True)
+    ref64 = oracle.density_contrast(pos, n, box, dtype=np.float64)
+    assert np.max(np.abs(d64 - ref64) / np.maximum(np.abs(ref64 + 1), 1.0)) < 1e-6
+    assert d32.max() > 1000.0  # the clump really is dense
 
 
 def test_density_contrast_matches_oracle(treepm, oracle):

@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import OMEGA_M, lattice, max_diff, max_norm, mesh_scattered, tree_scattered
+from helpers import OMEGA_M, fixed_point_delta32, lattice, max_diff, max_norm, mesh_scattered, tree_scattered
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -30,6 +30,18 @@ def test_deposit_matches_fp64_golden(oracle):
     scale = np.maximum(np.abs(ref), ref.mean())
     assert np.max(np.abs(g64 - ref) / scale) < 1e-12
     assert np.max(np.abs(g32 - ref) / scale) < 1e-5
+
+
+def test_fixed_point_delta_helper_matches_golden():
+    # The test helper that predicts the GPU's fixed-point density contrast bit for bit: against the
+    # numpy FP64 golden grid (FP32 weights, so to FP32 accuracy), exact mass conservation, and a
+    # uniform lattice giving delta = 0 exactly.
+    z = np.load(os.path.join(GOLD, "cic_5000.npz"))
+    d = fixed_point_delta32(z["pos"], 16, 1.0)
+    ref = z["grid"] / (0.25 * 5000 / 16 ** 3) - 1.0
+    assert np.max(np.abs(d - ref) / np.maximum(np.abs(ref + 1), 1.0)) < 1e-6
+    assert abs(((d.astype(np.float64) + 1).sum() - 16 ** 3)) < 1e-3
+    assert np.all(fixed_point_delta32(lattice(8, 1.0), 8, 1.0) == 0.0)
 
 
 def test_particle_at_cell_centre_fills_one_cell(oracle):
