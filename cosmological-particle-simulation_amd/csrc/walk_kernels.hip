@@ -342,7 +342,7 @@ __device__ __forceinline__ void eval_particles(WaveShared &sh, const float2 *sta
 }
 
 template <bool WRAP>
-__device__ __forceinline__ void eval_nodes(WaveShared &sh, int nn, const WalkArgs &w, double gx, double gy,
+__device__ __forceinline__ void eval_nodes(WaveShared &sh, const float2 *stab, int nn, const WalkArgs &w, double gx, double gy,
                                            double gz, float xi, float yi, float zi, bool active, float &ax,
                                            float &ay, float &az, uint32_t &ninter) {
     const int lane = threadIdx.x & 63;
@@ -389,9 +389,20 @@ __device__ __forceinline__ void eval_nodes(WaveShared &sh, int nn, const WalkArg
                 dy -= w.boxf * rintf(dy * w.invboxf);
                 dz -= w.boxf * rintf(dz * w.invboxf);
             }
-            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            // A node's centre of mass may lie beyond r_cut, outside the table: analytic S(r).
-            pair_f32(dx, dy, dz, r2, src.w, w, ax, ay, az);
+            // Within r_cut the pair table serves, scaled by the node's mass; a node's centre of mass
+            // may lie beyond r_cut, outside the table: there the analytic S(r).
+            const float s2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, w.eps2)));
+            if (s2 <= w.cut2s) {
+                const float2 t = stab[(int)s2];
+                const float rs = __builtin_amdgcn_rsqf(s2);
+                const float f = fmaf(t.x, rs * rs, t.y) * rs * src.w;
+                ax = fmaf(dx, f, ax);
+                ay = fmaf(dy, f, ay);
+                az = fmaf(dz, f, az);
+            } else {
+                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                pair_f32(dx, dy, dz, r2, src.w, w, ax, ay, az);
+            }
         }
         wave_sync();
     }
@@ -501,7 +512,7 @@ __device__ __forceinline__ void group_walk_body(WaveShared &sh, const float2 *st
             tot = 0;
         }
         if (nn > kNodeCap - 64) {
-            eval_nodes<WRAP>(sh, nn, w, gx, gy, gz, xi, yi, zi, active, ax, ay, az, ninter);
+            eval_nodes<WRAP>(sh, stab, nn, w, gx, gy, gz, xi, yi, zi, active, ax, ay, az, ninter);
             nn = 0;
         }
     }
@@ -510,7 +521,7 @@ __device__ __forceinline__ void group_walk_body(WaveShared &sh, const float2 *st
                                    ay, az, ninter, kept, tag);
         listed += (uint32_t)tot;
     }
-    if (nn > 0) eval_nodes<WRAP>(sh, nn, w, gx, gy, gz, xi, yi, zi, active, ax, ay, az, ninter);
+    if (nn > 0) eval_nodes<WRAP>(sh, stab, nn, w, gx, gy, gz, xi, yi, zi, active, ax, ay, az, ninter);
 }
 
 // Statistics of one wavefront's groups, summed in scalar registers and added to the global
