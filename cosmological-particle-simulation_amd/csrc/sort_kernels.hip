@@ -138,11 +138,12 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_up(K *__restrict__ keys
 }
 
 // One pass: keys_out/vals_out = keys/vals in stable order of the digit at `shift`; vals == null
-// means the identity (vals[i] = i). offsets = the scanned table.
-template <typename K>
+// means the identity (vals[i] = i). offsets = the scanned table. WIDE (the last pass over 32-bit
+// keys): keys_out receives the full 64-bit keys (the levels below 10 cleared).
+template <typename K, bool WIDE>
 __global__ void __launch_bounds__(kRadixThreads) k_radix_down(const K *__restrict__ keys, const uint32_t *__restrict__ vals,
                                                               size_t n, int shift, const uint32_t *__restrict__ offsets,
-                                                              uint32_t tiles, K *__restrict__ keys_out,
+                                                              uint32_t tiles, void *__restrict__ keys_out,
                                                               uint32_t *__restrict__ vals_out) {
     __shared__ uint32_t cnt[kRadixWaves][kRadixDigits];
     __shared__ uint32_t tile_off[kRadixDigits];
@@ -198,7 +199,10 @@ __global__ void __launch_bounds__(kRadixThreads) k_radix_down(const K *__restric
     for (uint32_t j = threadIdx.x; j < count; j += kRadixThreads) {
         const K k = skey[j];
         const size_t g = (size_t)gbase[radix_digit<K>(k, shift)] + j;
-        keys_out[g] = k;
+        if (WIDE)
+            static_cast<uint64_t *>(keys_out)[g] = full_key(k);
+        else
+            static_cast<K *>(keys_out)[g] = k;
         vals_out[g] = sval[j];
     }
 }
@@ -266,21 +270,24 @@ static void morton_sort(Ctx &c, const double *d_pos, size_t n, int lo, int hi) {
         TPM_HIP(device_scan(ScanLoad{c.radix_table}, ScanStore{c.radix_table}, entries, 0u, ScanSum{}, false,
                             c.scan_part, c.stream));
         const uint32_t *vin = first ? nullptr : va;
-        hipLaunchKernelGGL(k_radix_down<K>, dim3(tile_grid(n)), dim3(kRadixThreads), 0, c.stream, ka, vin, n, shift,
-                           c.radix_table, tiles, kb, vb);
+        // The last pass over 32-bit keys writes the full 64-bit keys (both buffers hold 64-bit keys).
+        if (sizeof(K) == 4 && shift + 8 >= hi)
+            hipLaunchKernelGGL((k_radix_down<K, true>), dim3(tile_grid(n)), dim3(kRadixThreads), 0, c.stream, ka, vin,
+                               n, shift, c.radix_table, tiles, (void *)kb, vb);
+        else
+            hipLaunchKernelGGL((k_radix_down<K, false>), dim3(tile_grid(n)), dim3(kRadixThreads), 0, c.stream, ka, vin,
+                               n, shift, c.radix_table, tiles, (void *)kb, vb);
         TPM_HIP(hipGetLastError());
         std::swap(ka, kb);
         std::swap(va, vb);
     }
-    // ka: the sorted sort keys. 64-bit keys are the full keys; 32-bit ones are widened into the other
-    // buffer by the gather.
-    const bool wide = sizeof(K) == 8;
-    uint64_t *full = reinterpret_cast<uint64_t *>(wide ? ka : kb);
+    // ka: the sorted keys, full 64-bit keys in either case.
+    uint64_t *full = reinterpret_cast<uint64_t *>(ka);
     hipLaunchKernelGGL(k_gather_positions<K>, dim3((unsigned)((n + 511) / 512)), dim3(256), 0, c.stream, d_pos, va,
-                       (const K *)ka, n, c.spos, wide ? (uint64_t *)nullptr : full);
+                       (const K *)nullptr, n, c.spos, (uint64_t *)nullptr);
     TPM_HIP(hipGetLastError());
     c.keys = full;
-    c.keys_alt = reinterpret_cast<uint64_t *>(wide ? kb : ka);
+    c.keys_alt = reinterpret_cast<uint64_t *>(kb);
     c.idx = va;
     c.idx_alt = vb;
 }
