@@ -4,9 +4,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 
 #include "device_scan.h"
+
 #include "tpm_internal.h"
 
 #include <algorithm>
@@ -81,7 +83,7 @@ static void free_all(Ctx &c) {
     for (hipEvent_t &e : c.walk_ev)
         if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {c.tw32, c.tw64, c.kvec, c.pos, c.acc, c.keys, c.keys_alt, c.idx, c.idx_alt, c.radix_table, c.scan_part, c.spos,
-                    c.counter, c.flag, c.status, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
+                    c.counter, c.flag, c.status, c.scan_part2, c.grid, c.specA, c.specB, c.specC, c.force, c.delta, c.dspecA, c.dspecB,
                     c.dspecC, c.dforce, c.tree.range, c.tree.link, c.tree.box, c.tree.com, c.tree_scan,
                     c.ngroups_dev, c.walk_ctr, c.walk_goff, c.dep_start, c.inv, c.judge_buf, c.gstart, c.gscratch, c.stab,
                     c.acc_sr, c.members, c.reject, c.mlp.params, c.mlp.grads, c.mlp.m1, c.mlp.m2};
@@ -93,6 +95,9 @@ static void free_all(Ctx &c) {
         if (p) (void)hipFree(p);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : {c.ev_fork, c.ev_join, c.tree_ev[0], c.tree_ev[1]})
+        if (e) (void)hipEventDestroy(e);
+    if (c.side) (void)hipStreamDestroy(c.side);
     if (c.stream) (void)hipStreamDestroy(c.stream);
 }
 
@@ -119,6 +124,7 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     c.radix_table_words = radix_tiles(mn) * 256;
     TPM_HIP(hipMalloc(&c.radix_table, c.radix_table_words * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.scan_part, kScanParts * sizeof(uint32_t)));
+    TPM_HIP(hipMalloc(&c.scan_part2, kScanParts * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.spos, 3 * mn * sizeof(double)));
     TPM_HIP(hipMalloc(&c.gstart, (mn + 1) * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.gscratch, 2 * mn * sizeof(uint32_t)));
@@ -195,16 +201,85 @@ static void read_tree_stats(Ctx &c, bool built) {
     c.stats.groups = ng;
 }
 
+// Switches the context's enqueues to its second stream, which first waits for everything enqueued
+// on the main stream so far (the sort), with the second set of scan partials; leave() switches back
+// (the second stream's work runs on, beside what the main stream does next) and join() makes the
+// main stream wait for it. The octree and the walk groups need only the sort, as do the deposit,
+// FFT and interpolation, so the two chains overlap and share no scratch buffer.
+struct SideStream {
+    Ctx &c;
+    hipStream_t main;
+    uint32_t *main_scan;
+    bool on = false;
+    explicit SideStream(Ctx &cc) : c(cc), main(cc.stream), main_scan(cc.scan_part) {
+        if (!c.side) {
+            // Made on the context's first large evaluation (the ensemble warms its boxes eagerly
+            // before a capture, so none is made during one).
+            TPM_HIP(hipStreamCreate(&c.side));
+            TPM_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+            TPM_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+        }
+        TPM_HIP(hipEventRecord(c.ev_fork, main));
+        TPM_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+        c.stream = c.side;
+        c.scan_part = c.scan_part2;
+        on = true;
+    }
+    void leave() {
+        TPM_HIP(hipEventRecord(c.ev_join, c.side));
+        c.stream = main;
+        c.scan_part = main_scan;
+        on = false;
+    }
+    void join() { TPM_HIP(hipStreamWaitEvent(main, c.ev_join, 0)); }
+    ~SideStream() {
+        if (on) {
+            c.stream = main;
+            c.scan_part = main_scan;
+        }
+    }
+};
+
+// Evaluations of more particles than this overlap the octree with the PM part. Smaller ones run
+// everything on one stream: there the kernels are short and latency-bound, and in an ensemble of
+// boxes, whose streams already overlap one another, the second streams made a 96-box run slower
+// (2.41 -> 2.61 s eager, 2.72 -> 3.50 s as a graph), while at 512^3 the overlap saves 1.2 ms.
+constexpr size_t kOverlapMin = size_t(1) << 22;
+
+// Octree and walk groups (the group walk's arguments), on whatever stream the context is on.
+static WalkArgs tree_chain(Ctx &c, size_t n, double k, float *d_acc, bool group_walk) {
+    if (c.timing) {
+        for (auto &e : c.tree_ev)
+            if (!e) TPM_HIP(hipEventCreate(&e));
+        TPM_HIP(hipEventRecord(c.tree_ev[0], c.stream));
+    }
+    build_tree(c, n, group_walk);
+    WalkArgs w{};
+    if (group_walk) w = walk_prepare(c, n, k, d_acc);
+    if (c.timing) TPM_HIP(hipEventRecord(c.tree_ev[1], c.stream));
+    return w;
+}
+
 // The force evaluation on device-resident positions (caller order, FP64).
 void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float *d_acc, float *d_delta) {
     TPM_REQUIRE(n <= c.max_n, "more particles than the context capacity");
     TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
     const bool treepm = (c.p.flags & TPM_FLAG_TREEPM) != 0;
+    const bool walk = treepm && n > 0;
+    const bool group_walk = walk && !(c.p.flags & TPM_FLAG_REF_WALK);
+    const double k = coupling(c, a, n, f32w);
     Timer tm(c);
     zero_fill(c, c.counter, 8 * sizeof(unsigned long long));
     tm.mark();
     sort_particles(c, d_pos, n);
     tm.mark();
+    WalkArgs wa{};
+    std::unique_ptr<SideStream> side;
+    if (walk && n > kOverlapMin) {
+        side.reset(new SideStream(c));
+        wa = tree_chain(c, n, k, d_acc, group_walk);
+        side->leave();
+    }
     deposit_fixed(c, n, 1.0, f32w, c.spos, true);
     tm.mark();
     if (n > 0) {
@@ -215,15 +290,16 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
     tm.mark();
     interp_force(c, n, f32w, d_acc);
     tm.mark();
-    if (treepm && n > 0) {
-        build_tree(c, n, !(c.p.flags & TPM_FLAG_REF_WALK));
-        tm.mark();
-        walk_short_range(c, n, coupling(c, a, n, f32w), d_acc);
-        tm.mark();
-    } else {
-        tm.mark();
-        tm.mark();
-    }
+    if (side)
+        side->join();
+    else if (walk)
+        wa = tree_chain(c, n, k, d_acc, group_walk);
+    tm.mark();
+    if (group_walk)
+        walk_launch(c, wa, n);
+    else if (walk)
+        walk_short_range(c, n, k, d_acc);
+    tm.mark();
     if (c.p.flags & TPM_FLAG_CHECK_FINITE) check_finite(c, d_acc, 3 * n);
     fold_status(c, treepm && n > 0);
     if (c.timing) {
@@ -232,7 +308,14 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
         c.stats.ms_deposit = tm.ms(1, 2);
         c.stats.ms_fft = tm.ms(2, 3);
         c.stats.ms_interp = tm.ms(3, 4);
-        c.stats.ms_tree = tm.ms(4, 5);
+        // The octree and walk groups (on the second stream beside the three stages above for large
+        // evaluations); the walk stage is the walk's launch.
+        c.stats.ms_tree = 0.0;
+        if (walk) {
+            float t = 0.f;
+            TPM_HIP(hipEventElapsedTime(&t, c.tree_ev[0], c.tree_ev[1]));
+            c.stats.ms_tree = t;
+        }
         c.stats.ms_walk = tm.ms(5, 6);
         c.stats.ms_total = tm.ms(0, 6);
         c.stats.ms_walk_kernel = 0.0;
@@ -252,8 +335,9 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
 }
 
 // evaluate() for FP64 positions, split around the short-range walk's launch so that the walks of
-// several boxes can share one launch: the same kernels in the same order (sort, deposit, PM,
-// interpolation into d_acc, octree, walk groups), then the prepared walk's arguments.
+// several boxes can share one launch: the same kernels on the same streams (sort; deposit, PM and
+// interpolation into d_acc; octree and walk groups, beside them on the second stream above
+// kOverlapMin particles), then the prepared walk's arguments.
 WalkArgs evaluate_pre_walk(Ctx &c, const double *d_pos, size_t n, double a, float *d_acc) {
     TPM_REQUIRE(n > 0 && n <= c.max_n, "bad particle count for a batched evaluation");
     TPM_REQUIRE(a > 0 && std::isfinite(a), "scale factor must be positive");
@@ -261,11 +345,21 @@ WalkArgs evaluate_pre_walk(Ctx &c, const double *d_pos, size_t n, double a, floa
                 "a batched evaluation needs the TreePM split and the group walk");
     zero_fill(c, c.counter, 8 * sizeof(unsigned long long));
     sort_particles(c, d_pos, n);
+    std::unique_ptr<SideStream> side;
+    WalkArgs w{};
+    if (n > kOverlapMin) {
+        side.reset(new SideStream(c));
+        w = tree_chain(c, n, coupling(c, a, n, false), d_acc, true);
+        side->leave();
+    }
     deposit_fixed(c, n, 1.0, false, c.spos, true);
     pm_from_grid(c, n, a, c.p.r_split, nullptr);
     interp_force(c, n, false, d_acc);
-    build_tree(c, n, true);
-    return walk_prepare(c, n, coupling(c, a, n, false), d_acc);
+    if (side)
+        side->join();
+    else
+        w = tree_chain(c, n, coupling(c, a, n, false), d_acc, true);
+    return w;
 }
 
 // The end of evaluate() after the walk: the finiteness check when asked and the status fold.

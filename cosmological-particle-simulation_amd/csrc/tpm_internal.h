@@ -110,6 +110,13 @@ struct Ctx {
     size_t m = 0;       // n^3
     size_t hs = 0;      // n * n * hp
     hipStream_t stream = nullptr;
+    // Second stream (made on first use): the octree and walk groups of a large evaluation run there
+    // beside the deposit, FFT and interpolation (both need only the sort), with scan partials of
+    // their own.
+    hipStream_t side = nullptr;
+    uint32_t *scan_part2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t tree_ev[2] = {nullptr, nullptr};  // around the tree chain when timing
     bool timing = false;
     bool count_pairs = true;  // walk interactions counted (tpm_set_counting)
     std::string err;
