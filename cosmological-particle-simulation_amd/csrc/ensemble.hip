@@ -253,19 +253,17 @@ static void tune_side_batched(tpm_ensemble &E, const std::vector<size_t> &sd, do
         float best = 1e30f;
         for (int r = 0; r < 2; ++r) {
             std::vector<WalkArgs> batch;
-            size_t total = 0;
             for (size_t q = 0; q < sd.size(); ++q) {
                 Box &bx = E.boxes[sd[q]];
                 Ctx &c = tpm_ctx_internal(bx.ctx);
                 batch.push_back(evaluate_pre_walk(c, bx.pos, bx.n, a, bx.acc));
-                total += bx.n;
                 if (q > 0) {
                     TPM_HIP(hipEventRecord(ev[q], c.stream));
                     TPM_HIP(hipStreamWaitEvent(lead.stream, ev[q], 0));
                 }
             }
             TPM_HIP(hipEventRecord(ev[0], lead.stream));
-            walk_launch_batch(lead, batch, total);
+            walk_launch_batch(lead, batch);
             TPM_HIP(hipEventRecord(ev[sd.size()], lead.stream));
             for (size_t q = 0; q < sd.size(); ++q) {
                 Box &bx = E.boxes[sd[q]];
@@ -533,13 +531,11 @@ int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start
                 for (const auto &sd : sides) {
                     Ctx &lead = tpm_ctx_internal(E.boxes[sd[0]].ctx);
                     std::vector<tpm::WalkArgs> batch;
-                    size_t total = 0;
                     for (size_t b : sd) {
                         if (b != sd[0]) TPM_HIP(hipStreamWaitEvent(lead.stream, pre[b], 0));
                         batch.push_back(wa[b]);
-                        total += E.boxes[b].n;
                     }
-                    tpm::walk_launch_batch(lead, batch, total);
+                    tpm::walk_launch_batch(lead, batch);
                     hipEvent_t walked = new_event();
                     TPM_HIP(hipEventRecord(walked, lead.stream));
                     for (size_t b : sd) {

@@ -292,7 +292,7 @@ void walk_launch(Ctx &c, const WalkArgs &w, size_t n);
 // One launch (per kWalkBatch boxes) walking the prepared walks of boxes that share every scalar
 // of their arguments (same split, softening, coupling, opening angle and small-node rule), on
 // lead.stream; each box's forces are those of its own walk_launch.
-void walk_launch_batch(Ctx &lead, const std::vector<WalkArgs> &boxes, size_t total_particles);
+void walk_launch_batch(Ctx &lead, const std::vector<WalkArgs> &boxes);
 
 // Short-range force of every particle added into acc[idx[i]] (group walk or reference walk).
 // With members != null only the `count` listed sorted particles are walked (all n are sources).

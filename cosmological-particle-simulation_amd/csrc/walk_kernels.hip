@@ -928,9 +928,16 @@ void walk_launch(Ctx &c, const WalkArgs &w, size_t walked) {
     if (c.timing) TPM_HIP(hipEventRecord(c.walk_ev[1], c.stream));
 }
 
-void walk_launch_batch(Ctx &lead, const std::vector<WalkArgs> &boxes, size_t total_particles) {
+void walk_launch_batch(Ctx &lead, const std::vector<WalkArgs> &boxes) {
     static_assert(sizeof(WalkArgs) + sizeof(WalkBoxArgs) <= 4096, "walk batch arguments exceed 4 KiB");
     TPM_REQUIRE(!boxes.empty(), "empty walk batch");
+    for (const WalkArgs &a : boxes) {
+        const WalkArgs &l = boxes[0];
+        TPM_REQUIRE(a.box == l.box && a.rcut == l.rcut && a.theta2 == l.theta2 && a.small_direct == l.small_direct &&
+                        a.scale == l.scale && a.eps2 == l.eps2 && a.cut2s == l.cut2s && a.inv2rs == l.inv2rs &&
+                        a.negk == l.negk && a.tabk == l.tabk,
+                    "the boxes of a batched walk must share every scalar of the walk");
+    }
     for (size_t b0 = 0; b0 < boxes.size(); b0 += kWalkBatch) {
         const size_t nb = std::min<size_t>(kWalkBatch, boxes.size() - b0);
         WalkBoxArgs bx{};
@@ -943,7 +950,6 @@ void walk_launch_batch(Ctx &lead, const std::vector<WalkArgs> &boxes, size_t tot
             bx.b[q] = WalkBox{a.spos, a.idx, a.gstart, a.ngroups, a.t, a.acc, a.counter};
             walked += a.n;
         }
-        (void)total_particles;
         WalkArgs w = boxes[b0];  // the shared scalars, the S table and the lead's counters
         w.next_group = lead.walk_ctr;
         w.counter = lead.counter;
