@@ -225,18 +225,26 @@ struct SideStream {
         c.scan_part = c.scan_part2;
         on = true;
     }
+    bool joined = false;
     void leave() {
         TPM_HIP(hipEventRecord(c.ev_join, c.side));
         c.stream = main;
         c.scan_part = main_scan;
         on = false;
     }
-    void join() { TPM_HIP(hipStreamWaitEvent(main, c.ev_join, 0)); }
+    void join() {
+        TPM_HIP(hipStreamWaitEvent(main, c.ev_join, 0));
+        joined = true;
+    }
+    // On an error between fork and join the main stream still waits for the second stream's work,
+    // so the next evaluation's sort cannot overwrite what an unfinished octree build reads.
     ~SideStream() {
         if (on) {
+            (void)hipEventRecord(c.ev_join, c.side);
             c.stream = main;
             c.scan_part = main_scan;
         }
+        if (!joined) (void)hipStreamWaitEvent(main, c.ev_join, 0);
     }
 };
 
@@ -267,7 +275,7 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
     const bool treepm = (c.p.flags & TPM_FLAG_TREEPM) != 0;
     const bool walk = treepm && n > 0;
     const bool group_walk = walk && !(c.p.flags & TPM_FLAG_REF_WALK);
-    const double k = coupling(c, a, n, f32w);
+    const double k = walk ? coupling(c, a, n, f32w) : 0.0;
     Timer tm(c);
     zero_fill(c, c.counter, 8 * sizeof(unsigned long long));
     tm.mark();
