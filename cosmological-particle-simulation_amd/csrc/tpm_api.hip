@@ -123,8 +123,10 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     TPM_HIP(hipMalloc(&c.idx_alt, mn * sizeof(uint32_t)));
     c.radix_table_words = radix_tiles(mn) * 256;
     TPM_HIP(hipMalloc(&c.radix_table, c.radix_table_words * sizeof(uint32_t)));
-    TPM_HIP(hipMalloc(&c.scan_part, kScanParts * sizeof(uint32_t)));
-    TPM_HIP(hipMalloc(&c.scan_part2, kScanParts * sizeof(uint32_t)));
+    TPM_HIP(hipMalloc(&c.scan_part, kLbSlotWords * sizeof(unsigned long long)));
+    TPM_HIP(hipMalloc(&c.scan_part2, kLbSlotWords * sizeof(unsigned long long)));
+    TPM_HIP(hipMemset(c.scan_part, 0, kLbSlotWords * sizeof(unsigned long long)));
+    TPM_HIP(hipMemset(c.scan_part2, 0, kLbSlotWords * sizeof(unsigned long long)));
     TPM_HIP(hipMalloc(&c.spos, 3 * mn * sizeof(double)));
     TPM_HIP(hipMalloc(&c.gstart, (mn + 1) * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.gscratch, 2 * mn * sizeof(uint32_t)));
@@ -209,7 +211,7 @@ static void read_tree_stats(Ctx &c, bool built) {
 struct SideStream {
     Ctx &c;
     hipStream_t main;
-    uint32_t *main_scan;
+    unsigned long long *main_scan;
     bool on = false;
     explicit SideStream(Ctx &cc) : c(cc), main(cc.stream), main_scan(cc.scan_part) {
         if (!c.side) {

@@ -111,10 +111,10 @@ struct Ctx {
     size_t hs = 0;      // n * n * hp
     hipStream_t stream = nullptr;
     // Second stream (made on first use): the octree and walk groups of a large evaluation run there
-    // beside the deposit, FFT and interpolation (both need only the sort), with scan partials of
+    // beside the deposit, FFT and interpolation (both need only the sort), with a look-back slot of
     // their own.
     hipStream_t side = nullptr;
-    uint32_t *scan_part2 = nullptr;
+    unsigned long long *scan_part2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t tree_ev[2] = {nullptr, nullptr};  // around the tree chain when timing
     bool timing = false;
@@ -134,7 +134,9 @@ struct Ctx {
     uint32_t *idx = nullptr, *idx_alt = nullptr;
     uint32_t *radix_table = nullptr;  // per-tile digit counts of a radix pass (256 per 4096 keys)
     size_t radix_table_words = 0;
-    uint32_t *scan_part = nullptr;    // partials of the device-wide scans (kScanParts words)
+    // Look-back slot of the single-pass scans and octree levels on the context's stream
+    // (kLbSlotWords words, zeroed at creation).
+    unsigned long long *scan_part = nullptr;
     double *spos = nullptr;    // 3 max_n positions in Morton order
     unsigned long long *counter = nullptr;  // interaction counter + scratch words
     int *flag = nullptr;       // non-finite flag

@@ -67,14 +67,10 @@ __device__ __forceinline__ void digit_bounds(const uint64_t *keys, uint32_t lo, 
 }
 
 // The octree is built level by level entirely on the device (no host synchronisation, so the
-// build can be captured in a HIP graph): lv[l] is the first node of level l, lv[l + 1] its end;
-// each level's child counts are scanned by a fixed-grid three-phase scan and the next level's
-// range is written to lv[l + 2].
-constexpr int kScanBlocks = 1024;
+// build can be captured in a HIP graph): lv[l] is the first node of level l, lv[l + 1] its end.
+// One kernel per level counts each node's children, places them by a single-pass (look-back) scan
+// of the counts and creates them; its last workgroup writes the next level's end to lv[l + 2].
 
-// Pass 1 over one level: decide which nodes split and count their non-empty octants.
-// Nodes of at most `keep_whole` particles that would split get no children (the group walk never
-// opens them) but keep their walk-leaf status from the split they would have.
 // Octant boundaries of a node of `level` (cube prefix of its first key) from the table of the first
 // sorted index of every level-`tab_level` cube, for level < tab_level: no search at all.
 __device__ __forceinline__ void table_bounds(const uint64_t *keys, const uint32_t *tab, int tab_level, uint32_t first,
@@ -85,158 +81,128 @@ __device__ __forceinline__ void table_bounds(const uint64_t *keys, const uint32_
     for (int o = 0; o <= 8; ++o) b[o] = tab[(size_t)((cube << 3) + (uint64_t)o) << sh];
 }
 
-__global__ void __launch_bounds__(256) k_count_children(const uint64_t *__restrict__ keys, TreeView t,
-                                                        const uint32_t *__restrict__ lv, int level, int max_depth,
-                                                        uint32_t ref_leaf_rule, uint32_t keep_whole,
-                                                        const uint32_t *__restrict__ tab, int tab_level,
-                                                        uint32_t *__restrict__ nchild) {
-    const uint32_t lb = lv[level], le = lv[level + 1];
-    for (uint32_t q = lb + blockIdx.x * blockDim.x + threadIdx.x; q < le; q += gridDim.x * blockDim.x) {
-        const uint2 r = t.range[q];
-        const uint32_t cnt = r.y - r.x;
-        uint32_t mask = 0, nc = 0;
-        if (cnt > (uint32_t)kLeafSize && level < max_depth) {
-            uint32_t b[9];
-            if (level < tab_level)
-                table_bounds(keys, tab, tab_level, r.x, level, b);
-            else
-                digit_bounds(keys, r.x, r.y, level + 1, b);
-            for (int o = 0; o < 8; ++o)
-                if (b[o + 1] > b[o]) {
-                    mask |= 1u << o;
-                    ++nc;
-                }
-        }
-        // Walk-leaf: no children, or (reference rule) no child in octant 0 (src/tree.rs:280).
-        const uint32_t walk_leaf = (nc == 0 || (ref_leaf_rule && !(mask & 1u))) ? 1u : 0u;
-        if (cnt <= keep_whole) nc = 0;
-        uint4 l = t.link[q];
-        l.y = nc;
-        l.z = mask;
-        l.w = (uint32_t)level | (walk_leaf << 8);
+// Pass 1 for node q of `level`: decide whether it splits, count its non-empty octants and record
+// count, octant mask, level and walk-leaf status in its link (returns the count). Nodes of at most
+// `keep_whole` particles that would split get no children (the group walk never opens them) but
+// keep their walk-leaf status from the split they would have.
+__device__ __forceinline__ uint32_t count_node(const uint64_t *__restrict__ keys, TreeView t, uint32_t q, int level,
+                                               int max_depth, uint32_t ref_leaf_rule, uint32_t keep_whole,
+                                               const uint32_t *__restrict__ tab, int tab_level) {
+    const uint2 r = t.range[q];
+    const uint32_t cnt = r.y - r.x;
+    uint32_t mask = 0, nc = 0;
+    if (cnt > (uint32_t)kLeafSize && level < max_depth) {
+        uint32_t b[9];
+        if (level < tab_level)
+            table_bounds(keys, tab, tab_level, r.x, level, b);
+        else
+            digit_bounds(keys, r.x, r.y, level + 1, b);
+        for (int o = 0; o < 8; ++o)
+            if (b[o + 1] > b[o]) {
+                mask |= 1u << o;
+                ++nc;
+            }
+    }
+    // Walk-leaf: no children, or (reference rule) no child in octant 0 (src/tree.rs:280).
+    const uint32_t walk_leaf = (nc == 0 || (ref_leaf_rule && !(mask & 1u))) ? 1u : 0u;
+    if (cnt <= keep_whole) nc = 0;
+    uint4 l = t.link[q];
+    l.y = nc;
+    l.z = mask;
+    l.w = (uint32_t)level | (walk_leaf << 8);
+    t.link[q] = l;
+    return nc;
+}
+
+// Pass 2 for node q: create its children at [first, first + count) (count from pass 1).
+__device__ __forceinline__ void make_children(const uint64_t *__restrict__ keys, TreeView t, uint32_t q, int level,
+                                              uint64_t first, uint32_t cap, const uint32_t *__restrict__ tab,
+                                              int tab_level) {
+    uint4 l = t.link[q];
+    if (l.y == 0) {
+        l.x = 0xFFFFFFFFu;
         t.link[q] = l;
-        nchild[q - lb] = nc;
+        return;
+    }
+    if (first + l.y > cap) {
+        // Over capacity (reported through info[1]): the node stays a leaf so the walk stays exact.
+        l.x = 0xFFFFFFFFu;
+        l.y = 0;
+        l.w |= 1u << 8;
+        t.link[q] = l;
+        return;
+    }
+    const uint2 r = t.range[q];
+    uint32_t b[9];
+    if (level < tab_level)
+        table_bounds(keys, tab, tab_level, r.x, level, b);
+    else
+        digit_bounds(keys, r.x, r.y, level + 1, b);
+    l.x = (uint32_t)first;
+    t.link[q] = l;
+    const float4 pb = t.box[q];
+    const float quarter = pb.w / 2.0f;
+    uint32_t ci = (uint32_t)first;
+    for (int o = 0; o < 8; ++o) {
+        if (b[o + 1] == b[o]) continue;
+        t.range[ci] = make_uint2(b[o], b[o + 1]);
+        // src/tree.rs:243-245: centre[k] +- quarter in FP32.
+        t.box[ci] = make_float4(pb.x + ((o & 1) ? quarter : -quarter), pb.y + ((o & 2) ? quarter : -quarter),
+                                pb.z + ((o & 4) ? quarter : -quarter), quarter);
+        t.link[ci] = make_uint4(0xFFFFFFFFu, 0, 0, (uint32_t)(level + 1));
+        ++ci;
     }
 }
 
-// Chunk [c0, c1) of a scan over cnt values handled by block b of kScanBlocks.
-__device__ inline void scan_chunk(uint32_t cnt, uint32_t b, uint32_t &c0, uint32_t &c1) {
-    const uint32_t per = (cnt + kScanBlocks - 1) / kScanBlocks;
-    c0 = min(cnt, b * per);
-    c1 = min(cnt, c0 + per);
-}
-
-__global__ void __launch_bounds__(256) k_scan_partials(const uint32_t *__restrict__ in, const uint32_t *__restrict__ lv,
-                                                       int level, uint32_t *__restrict__ part) {
-    __shared__ uint32_t sh[256];
-    const uint32_t cnt = lv[level + 1] - lv[level];
-    uint32_t c0, c1;
-    scan_chunk(cnt, blockIdx.x, c0, c1);
-    uint32_t s = 0;
-    for (uint32_t i = c0 + threadIdx.x; i < c1; i += 256) s += in[i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
+// One level of the build: each of the G workgroups takes a contiguous chunk of the level's nodes
+// (in 256-node tiles), counts their children (pass 1), obtains the number of children of all
+// earlier chunks by look-back (`slot`, see device_scan.h), then creates its nodes' children in node
+// order from there (pass 2). The grid is sized by the level's largest possible node count, so
+// most workgroups of a deep level usually have no chunk: only the first `chunks` workgroups take a
+// ticket (at least one, when the level is empty), the others end at once. The workgroup of the
+// last chunk writes the next level's end (clamped to the capacity, with info[1] set when children
+// had to be dropped) to lv[level + 2] and info[0].
+__global__ void __launch_bounds__(256) k_build_level(const uint64_t *__restrict__ keys, TreeView t,
+                                                     uint32_t *__restrict__ lv, int level, int max_depth,
+                                                     uint32_t ref_leaf_rule, uint32_t keep_whole,
+                                                     const uint32_t *__restrict__ tab, int tab_level, uint32_t cap,
+                                                     uint32_t *__restrict__ info, unsigned long long *__restrict__ slot,
+                                                     uint32_t G) {
+    __shared__ uint32_t sh4[4];
+    const uint32_t lb = lv[level], le = lv[level + 1];
+    const uint32_t tiles = (le - lb + 255) / 256;
+    const uint32_t per = tiles ? (tiles + G - 1) / G : 1u;
+    const uint32_t chunks = tiles ? (tiles + per - 1) / per : 1u;  // non-empty chunks (one if none)
+    if (blockIdx.x >= chunks) return;  // uniform across the workgroup; takes no ticket
+    uint32_t b, ep;
+    lb_ticket(slot, chunks, b, ep);
+    const uint32_t t0 = min(tiles, b * per), t1 = min(tiles, t0 + per);
+    const uint32_t last = chunks - 1;
+    uint32_t mine = 0;
+    for (uint32_t tile = t0; tile < t1; ++tile) {
+        const uint32_t q = lb + tile * 256 + threadIdx.x;
+        if (q < le) mine += count_node(keys, t, q, level, max_depth, ref_leaf_rule, keep_whole, tab, tab_level);
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
-// Exclusive scan of the kScanBlocks partials (one block); the total gives the next level's end.
-__global__ void __launch_bounds__(kScanBlocks) k_scan_top(uint32_t *__restrict__ part, uint32_t *__restrict__ lv,
-                                                          int level, uint32_t cap, uint32_t *__restrict__ info) {
-    __shared__ uint32_t sh[kScanBlocks];
-    const uint32_t v = part[threadIdx.x];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int d = 1; d < kScanBlocks; d <<= 1) {
-        const uint32_t add = (int)threadIdx.x >= d ? sh[threadIdx.x - d] : 0u;
-        __syncthreads();
-        sh[threadIdx.x] += add;
-        __syncthreads();
+    uint32_t total;
+    (void)block_exclusive_256(mine, 0u, ScanSum{}, total, sh4);
+    uint32_t carry = lb_prefix(slot + 1, b, ep, total, 0u, ScanSum{});
+    for (uint32_t tile = t0; tile < t1; ++tile) {
+        const uint32_t q = lb + tile * 256 + threadIdx.x;
+        // This thread wrote node q's count in pass 1.
+        const uint32_t nc = q < le ? t.link[q].y : 0u;
+        uint32_t tile_total;
+        const uint32_t ex = block_exclusive_256(nc, 0u, ScanSum{}, tile_total, sh4);
+        if (q < le) make_children(keys, t, q, level, (uint64_t)le + carry + ex, cap, tab, tab_level);
+        carry += tile_total;
     }
-    part[threadIdx.x] = sh[threadIdx.x] - v;
-    if (threadIdx.x == kScanBlocks - 1) {
-        const uint32_t le = lv[level + 1];
-        uint64_t end = (uint64_t)le + sh[threadIdx.x];
+    if (b == last && threadIdx.x == 0) {
+        uint64_t end = (uint64_t)le + carry;
         if (end > cap) {
             info[1] = 1u;  // capacity exceeded: children past the capacity are not created
             end = cap;
         }
         lv[level + 2] = (uint32_t)end;
         info[0] = (uint32_t)end;
-    }
-}
-
-// Exclusive scan within each block's chunk, offset by the block's partial.
-__global__ void __launch_bounds__(256) k_scan_finish(const uint32_t *__restrict__ in, const uint32_t *__restrict__ lv,
-                                                     int level, const uint32_t *__restrict__ part,
-                                                     uint32_t *__restrict__ out) {
-    __shared__ uint32_t sh[256];
-    const uint32_t cnt = lv[level + 1] - lv[level];
-    uint32_t c0, c1;
-    scan_chunk(cnt, blockIdx.x, c0, c1);
-    uint32_t carry = part[blockIdx.x];
-    for (uint32_t t0 = c0; t0 < c1; t0 += 256) {
-        const uint32_t i = t0 + threadIdx.x;
-        const uint32_t v = i < c1 ? in[i] : 0u;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (int d = 1; d < 256; d <<= 1) {
-            const uint32_t add = (int)threadIdx.x >= d ? sh[threadIdx.x - d] : 0u;
-            __syncthreads();
-            sh[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (i < c1) out[i] = carry + sh[threadIdx.x] - v;
-        carry += sh[255];
-        __syncthreads();
-    }
-}
-
-// Pass 2: create the children of each split node at first = le + offset[q - lb].
-__global__ void __launch_bounds__(256) k_make_children(const uint64_t *__restrict__ keys, TreeView t,
-                                                       const uint32_t *__restrict__ lv, int level,
-                                                       const uint32_t *__restrict__ offset, uint32_t cap,
-                                                       const uint32_t *__restrict__ tab, int tab_level) {
-    const uint32_t lb = lv[level], le = lv[level + 1];
-    for (uint32_t q = lb + blockIdx.x * blockDim.x + threadIdx.x; q < le; q += gridDim.x * blockDim.x) {
-        uint4 l = t.link[q];
-        if (l.y == 0) {
-            l.x = 0xFFFFFFFFu;
-            t.link[q] = l;
-            continue;
-        }
-        const uint2 r = t.range[q];
-        uint32_t b[9];
-        if (level < tab_level)
-            table_bounds(keys, tab, tab_level, r.x, level, b);
-        else
-            digit_bounds(keys, r.x, r.y, level + 1, b);
-        const uint64_t first = (uint64_t)le + offset[q - lb];
-        if (first + l.y > cap) {
-            // Over capacity (reported through info[1]): the node stays a leaf so the walk stays exact.
-            l.x = 0xFFFFFFFFu;
-            l.y = 0;
-            l.w |= 1u << 8;
-            t.link[q] = l;
-            continue;
-        }
-        l.x = (uint32_t)first;
-        t.link[q] = l;
-        const float4 pb = t.box[q];
-        const float quarter = pb.w / 2.0f;
-        uint32_t ci = (uint32_t)first;
-        for (int o = 0; o < 8; ++o) {
-            if (b[o + 1] == b[o]) continue;
-            t.range[ci] = make_uint2(b[o], b[o + 1]);
-            // src/tree.rs:243-245: centre[k] +- quarter in FP32.
-            t.box[ci] = make_float4(pb.x + ((o & 1) ? quarter : -quarter), pb.y + ((o & 2) ? quarter : -quarter),
-                                    pb.z + ((o & 4) ? quarter : -quarter), quarter);
-            t.link[ci] = make_uint4(0xFFFFFFFFu, 0, 0, (uint32_t)(level + 1));
-            ++ci;
-        }
     }
 }
 
@@ -356,9 +322,8 @@ static void ensure_tree(Ctx &c, size_t cap) {
     TPM_HIP(hipMalloc(&t.link, cap * sizeof(uint4)));
     TPM_HIP(hipMalloc(&t.box, cap * sizeof(float4)));
     TPM_HIP(hipMalloc(&t.com, cap * sizeof(double4)));
-    // child counts and offsets (cap each), scan partials, level table and info words
-    // child counts and offsets (cap each), scan partials, level table, info words, cube table (< cap)
-    TPM_HIP(hipMalloc(&c.tree_scan, (3 * cap + kScanBlocks + kMaxDepth + 8) * sizeof(uint32_t)));
+    // scratch of the cube table's scan (cap), level table, info words, cube table (< cap)
+    TPM_HIP(hipMalloc(&c.tree_scan, (2 * cap + kMaxDepth + 11) * sizeof(uint32_t)));
     t.cap = cap;
 }
 
@@ -368,10 +333,8 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
     Tree &t = c.tree;
     const int depth = sort_depth(c, n);  // the levels the sort ordered
     ensure_tree(c, tree_capacity(n));
-    uint32_t *nch = c.tree_scan;
-    uint32_t *off = nch + t.cap;
-    uint32_t *part = off + t.cap;
-    uint32_t *lv = part + kScanBlocks;
+    uint32_t *off = c.tree_scan;
+    uint32_t *lv = off + t.cap;
     uint32_t *info = lv + kMaxDepth + 3;
     c.tree_info = info;
     // Root: the whole box, centre L/2 and half L/2 in FP32 (src/tree.rs:180-181).
@@ -407,15 +370,11 @@ void build_tree(Ctx &c, size_t n, bool group_walk_only) {
         return (unsigned)std::max(1.0, std::min(std::ceil(bound / 256.0), 65536.0));
     };
     for (int level = 0; level <= depth; ++level) {
-        const unsigned grid = level_grid(level);
-        hipLaunchKernelGGL(k_count_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level,
-                           depth, ref_rule, keep_whole, tab, tab_level, nch);
-        hipLaunchKernelGGL(k_scan_partials, dim3(kScanBlocks), dim3(256), 0, c.stream, nch, lv, level, part);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlocks), 0, c.stream, part, lv, level, (uint32_t)t.cap,
-                           info);
-        hipLaunchKernelGGL(k_scan_finish, dim3(kScanBlocks), dim3(256), 0, c.stream, nch, lv, level, part, off);
-        hipLaunchKernelGGL(k_make_children, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level, off,
-                           (uint32_t)t.cap, tab, tab_level);
+        // One workgroup per 256 possible nodes, at most kLbMaxBlocks (larger levels: several tiles
+        // per workgroup).
+        const unsigned grid = std::min<unsigned>(level_grid(level), kLbMaxBlocks);
+        hipLaunchKernelGGL(k_build_level, dim3(grid), dim3(256), 0, c.stream, c.keys, t.view(), lv, level, depth,
+                           ref_rule, keep_whole, tab, tab_level, (uint32_t)t.cap, info, c.scan_part, grid);
         TPM_HIP(hipGetLastError());
     }
     // Centres of mass, deepest level first (levels past the deepest are empty ranges).
