@@ -80,6 +80,15 @@ __device__ __forceinline__ void cic_stencil(T x0, T x1, T x2, int n, T box, Sten
     cic_corners<T>(base, frac, n, st);
 }
 
+// Quantum of one corner: the nearest integer (ties to even, as llrint) of x = w * mass * 2^32.
+// When 0 <= mass * 2^32 < 2^52 (`small`: uniform across the launch), x + 2^52 is that integer
+// plus 2^52, exactly, in the low mantissa bits of the FP64 sum: one addition and one 64-bit
+// subtraction instead of llrint's conversion sequence, with the same result.
+__device__ __forceinline__ unsigned long long quanta(double x, bool small) {
+    if (small) return (unsigned long long)__double_as_longlong(x + 0x1p52) - 0x4330000000000000ull;
+    return (unsigned long long)llrint(x);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_deposit(const double *__restrict__ pos, size_t np, int n, double box,
                                                  double mass, unsigned long long *__restrict__ grid) {
@@ -88,11 +97,12 @@ __global__ void __launch_bounds__(256) k_deposit(const double *__restrict__ pos,
     Stencil<T> st;
     cic_stencil<T>((T)pos[3 * i], (T)pos[3 * i + 1], (T)pos[3 * i + 2], n, (T)box, st);
     const double scale = mass * 0x1p32;
+    const bool small = scale >= 0.0 && scale < 0x1p52;
     const unsigned long long total = (unsigned long long)llrint(scale);
     unsigned long long used = 0;
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
-        const unsigned long long q = (unsigned long long)llrint((double)st.w[c] * scale);
+        const unsigned long long q = quanta((double)st.w[c] * scale, small);
         used += q;
         atomicAdd(grid + st.idx[c], q);
     }
@@ -101,16 +111,24 @@ __global__ void __launch_bounds__(256) k_deposit(const double *__restrict__ pos,
     atomicAdd(grid + st.idx[7], total - used);
 }
 
-// Same quanta as k_deposit, accumulated per workgroup in an LDS tile first. A workgroup takes
-// up to kDepItem particles of one aligned Morton cube of the sorted order (cubes of at most 8
-// cells on a side, so the CIC footprint, with a margin of two cells for the rounding of the
-// keys, always fits the tile; dense cubes are split into several items), adds their 8 quanta
-// with LDS atomics and then adds each non-zero tile cell to the grid with one global atomic. A
-// particle whose stencil lies outside the tile (only possible for a position outside [0, L))
-// adds directly to the grid. Integer sums commute, so the grid is bit-identical to k_deposit's
-// for any order and any split.
-constexpr int kDepTile = 13 * 13 * 13;
-constexpr uint32_t kDepItem = 2048;
+// Same quanta as k_deposit, accumulated per workgroup in an LDS tile first. A workgroup of
+// kDepThreads takes up to kDepItem particles of one aligned Morton cube of the sorted order
+// (cubes of at most kDepSide cells on a side, so the CIC footprint, with a margin of two cells for
+// the rounding of the keys, always fits the tile; dense cubes are split into several items), adds
+// their 8 quanta with LDS atomics and then adds each non-zero tile cell to the grid with one
+// global atomic. A particle whose stencil lies outside the tile (only possible for a position
+// outside [0, L)) adds directly to the grid. Integer sums commute, so the grid is bit-identical to
+// k_deposit's for any order and any split. A cube of 16 cells a side holds about 4,096 particles
+// at the mean density: its tile of at most 21^3 cells (74 KB, two workgroups per CU) is cleared
+// and flushed once per 4,096 particles, about 1.7 tile cells per particle against 3.4 for cubes of
+// 8 cells, and the workgroup's fixed costs (the cube's bounds, the flush's reciprocals) are spread
+// over eight times as many particles.
+constexpr int kDepTileSide = kDepSide + 5;  // 2 cells below, 1 above, 1 for a ceil, 1 stencil corner
+constexpr int kDepTile = kDepTileSide * kDepTileSide * kDepTileSide;
+constexpr int kDepThreads = 1024;
+constexpr int kDepBatch = 2;
+constexpr uint32_t kDepItem = 2u * kDepSide * kDepSide * kDepSide;  // 8 particles per thread
+constexpr size_t kDepExtra = 1024;  // workgroups for the dense cubes' extra items
 
 // First sorted index of every level-`level` Morton cube (starts[nodes] = np).
 __global__ void k_node_starts(const uint64_t *__restrict__ keys, size_t np, int shift, uint32_t nodes,
@@ -148,16 +166,16 @@ __device__ __forceinline__ uint32_t morton_axis(uint32_t k, int axis, int level)
     return v;
 }
 
+// One work item of the deposit, by the whole workgroup (uniform control flow; the tile is free
+// again when it returns). Item b < nodes is the first item of cube b; item nodes + e is extra item
+// e of the dense cubes: extra item e belongs to the cube k with item_start[k] <= e <
+// item_start[k + 1] (item_start: exclusive scan of the extra items per cube), as its item
+// 1 + e - item_start[k].
 template <typename T>
-__global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__ pos, const uint32_t *__restrict__ starts,
-                                                      const uint32_t *__restrict__ item_start, uint32_t nodes,
-                                                      int level, int n, double box, double mass,
-                                                      unsigned long long *__restrict__ grid) {
-    __shared__ unsigned long long tile[kDepTile];
-    // Workgroup b < nodes takes the first item of cube b; workgroup nodes + e takes extra item e of
-    // the dense cubes: extra item e belongs to the cube k with item_start[k] <= e < item_start[k + 1]
-    // (item_start: exclusive scan of the extra items per cube), as its item 1 + e - item_start[k].
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void deposit_item(uint32_t b, unsigned long long *tile, const double *__restrict__ pos,
+                                             const uint32_t *__restrict__ starts,
+                                             const uint32_t *__restrict__ item_start, uint32_t nodes, int level,
+                                             int n, double box, double mass, unsigned long long *__restrict__ grid) {
     uint32_t k, sub;
     if (b < nodes) {
         k = b;
@@ -192,18 +210,19 @@ __global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__
         dim[a] = hi - lo[a] + 1;
     }
     const int cells = dim[0] * dim[1] * dim[2];
-    for (int c = tid; c < cells; c += 256) tile[c] = 0ull;
+    for (int c = tid; c < cells; c += kDepThreads) tile[c] = 0ull;
     __syncthreads();
     const double scale = mass * 0x1p32;
+    const bool small = scale >= 0.0 && scale < 0x1p52;
     const unsigned long long total = (unsigned long long)llrint(scale);
     if (starts[k + 1] - starts[k] > kDepItem) {
-        // A dense cube (more than one item: at least four particles per cell on average). Each
+        // A dense cube (more than one item: at least two particles per cell on average). Each
         // thread takes a contiguous run of the sorted order, whose consecutive particles in dense
         // regions share their CIC base cell, and sums their quanta per corner in registers, adding
         // them to the tile only when the base cell changes: the LDS atomics of particles crowded
         // into few cells, which serialise on equal addresses, become one per run. Integer sums:
         // the grid is bit-identical to adding every quantum on its own.
-        const uint32_t run = (p1 - p0 + 255) / 256;
+        const uint32_t run = (p1 - p0 + kDepThreads - 1) / kDepThreads;
         const uint32_t q0 = p0 + tid * run, q1 = min(p1, q0 + run);
         const int s1 = dim[2], s0 = dim[1] * dim[2];
         const int corner[8] = {0, 1, s1, s1 + 1, s0, s0 + 1, s0 + s1, s0 + s1 + 1};
@@ -241,7 +260,7 @@ __global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__
                 const T wy = dy == 0 ? (T)1 - frac[1] : frac[1];
                 const T wz = dz == 0 ? (T)1 - frac[2] : frac[2];
                 const T w = (wx * wy) * wz;
-                const unsigned long long q = c < 7 ? (unsigned long long)llrint((double)w * scale) : total - used;
+                const unsigned long long q = c < 7 ? quanta((double)w * scale, small) : total - used;
                 used += q;
                 if (inside)
                     sum[c] += q;
@@ -253,14 +272,16 @@ __global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__
 #pragma unroll
             for (int c = 0; c < 8; ++c) atomicAdd(&tile[cur + corner[c]], sum[c]);
     } else {
-        for (uint32_t i = p0 + tid; i < p1; i += 256) {
+        const int s1 = dim[2], s0 = dim[1] * dim[2];
+        auto deposit_one = [&](double x0, double x1, double x2) {
             int64_t base[3];
             T frac[3];
-    #pragma unroll
-            for (int a = 0; a < 3; ++a) cic_axis<T>((T)pos[3 * (size_t)i + a], h, base[a], frac[a]);
+            cic_axis<T>((T)x0, h, base[0], frac[0]);
+            cic_axis<T>((T)x1, h, base[1], frac[1]);
+            cic_axis<T>((T)x2, h, base[2], frac[2]);
             bool inside = true;
             int off[3];
-    #pragma unroll
+#pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const int64_t o = base[a] - lo[a];
                 inside = inside && o >= 0 && o <= dim[a] - 2;
@@ -268,31 +289,49 @@ __global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__
             }
             Stencil<T> st;
             if (!inside) cic_corners<T>(base, frac, n, st);
+            const int t0 = (off[0] * dim[1] + off[1]) * dim[2] + off[2];
             unsigned long long used = 0;
-    #pragma unroll
+#pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int dx = (c >> 2) & 1, dy = (c >> 1) & 1, dz = c & 1;
                 const T wx = dx == 0 ? (T)1 - frac[0] : frac[0];
                 const T wy = dy == 0 ? (T)1 - frac[1] : frac[1];
                 const T wz = dz == 0 ? (T)1 - frac[2] : frac[2];
                 const T w = (wx * wy) * wz;
-                const unsigned long long q = c < 7 ? (unsigned long long)llrint((double)w * scale) : total - used;
+                const unsigned long long q = c < 7 ? quanta((double)w * scale, small) : total - used;
                 used += q;
                 if (inside)
-                    atomicAdd(&tile[((off[0] + dx) * dim[1] + (off[1] + dy)) * dim[2] + (off[2] + dz)], q);
+                    atomicAdd(&tile[t0 + dx * s0 + dy * s1 + dz], q);
                 else
                     atomicAdd(grid + st.idx[c], q);
             }
+        };
+        // kDepBatch particles per thread and round (a cube at the mean density is one round), all
+        // loaded before any is deposited, so that their global loads are in flight together (the
+        // indices past the item clamped into it).
+        for (uint32_t i = p0 + tid; i < p1; i += kDepBatch * kDepThreads) {
+            double x[kDepBatch][3];
+#pragma unroll
+            for (int u = 0; u < kDepBatch; ++u) {
+                const size_t j = min(i + u * kDepThreads, p1 - 1);
+                x[u][0] = pos[3 * j];
+                x[u][1] = pos[3 * j + 1];
+                x[u][2] = pos[3 * j + 2];
+            }
+#pragma unroll
+            for (int u = 0; u < kDepBatch; ++u)
+                if (i + u * kDepThreads < p1) deposit_one(x[u][0], x[u][1], x[u][2]);
         }
     }
     __syncthreads();
     // Tile cell c -> (ix, iy, iz) by multiply-high with per-workgroup reciprocals m = ceil(2^32 / d),
-    // exact here (c * (m d - 2^32) < c d < 2^32 for c < 13^3); the tile spans at most 13 cells per
-    // axis, so for meshes of 16 cells or more one conditional add or subtract wraps it.
+    // exact here (c * (m d - 2^32) < c d < 2^32 for c < 21^3, d <= 21^2); the tile spans at most 21
+    // cells per axis, from 2 below the mesh, so for meshes of 32 cells or more one conditional add
+    // or subtract wraps it.
     const uint32_t d12 = (uint32_t)(dim[1] * dim[2]), d2 = (uint32_t)dim[2];
     const uint32_t m12 = (uint32_t)((0x100000000ull + d12 - 1) / d12), m2 = (uint32_t)((0x100000000ull + d2 - 1) / d2);
-    const bool small_mesh = n < 16;
-    for (int c = tid; c < cells; c += 256) {
+    const bool small_mesh = n < 32;
+    for (int c = tid; c < cells; c += kDepThreads) {
         const unsigned long long v = tile[c];
         if (!v) continue;
         const uint32_t ix = __umulhi((uint32_t)c, m12);
@@ -311,6 +350,28 @@ __global__ void __launch_bounds__(256) k_deposit_cube(const double *__restrict__
         }
         atomicAdd(grid + (size_t)(gx * n + gy) * (size_t)n + (size_t)gz, v);
     }
+    __syncthreads();  // the tile's reads are done before the next item clears it
+}
+
+// Workgroup b < nodes takes the first item of cube xcd_block(b, nodes) (each XCD a contiguous
+// range of cubes); the remaining workgroups take the dense cubes' extra items in turn, as many as
+// the device-side count says (none for an input without dense cubes), so the launch needs no
+// workgroup per possible extra item.
+template <typename T>
+__global__ void __launch_bounds__(kDepThreads) k_deposit_cube(const double *__restrict__ pos,
+                                                              const uint32_t *__restrict__ starts,
+                                                              const uint32_t *__restrict__ item_start, uint32_t nodes,
+                                                              int level, int n, double box, double mass,
+                                                              unsigned long long *__restrict__ grid) {
+    __shared__ unsigned long long tile[kDepTile];
+    const uint32_t b = blockIdx.x;
+    if (b < nodes) {
+        deposit_item<T>(xcd_block(b, nodes), tile, pos, starts, item_start, nodes, level, n, box, mass, grid);
+        return;
+    }
+    const uint32_t extra = item_start[nodes];
+    for (uint32_t e = b - nodes; e < extra; e += gridDim.x - nodes)
+        deposit_item<T>(nodes + e, tile, pos, starts, item_start, nodes, level, n, box, mass, grid);
 }
 
 __global__ void k_grid_to_float(const unsigned long long *__restrict__ grid, size_t m, float mean32,
@@ -477,14 +538,15 @@ void deposit_fixed(Ctx &c, size_t n, double mass, bool f32_weights, const double
         hipLaunchKernelGGL(k_node_items, dim3(blocks(nodes + 1, 256)), dim3(256), 0, c.stream, c.dep_start, nodes,
                            items);
         exclusive_sum_u32(c, items, item_start, nodes + 1);
-        // One workgroup per cube for its first item, plus at most n / kDepItem extra items.
-        const size_t grid_items = (size_t)nodes + n / kDepItem + 1;
+        // One workgroup per cube for its first item, plus up to kDepExtra for the at most
+        // n / kDepItem extra items.
+        const size_t grid_items = (size_t)nodes + std::min<size_t>(n / kDepItem + 1, kDepExtra);
         if (f32_weights)
-            hipLaunchKernelGGL(k_deposit_cube<float>, dim3((unsigned)grid_items), dim3(256), 0, c.stream, pos_src,
-                               c.dep_start, item_start, nodes, c.dep_level, c.n, c.p.box_size, mass, c.grid);
+            hipLaunchKernelGGL(k_deposit_cube<float>, dim3((unsigned)grid_items), dim3(kDepThreads), 0, c.stream,
+                               pos_src, c.dep_start, item_start, nodes, c.dep_level, c.n, c.p.box_size, mass, c.grid);
         else
-            hipLaunchKernelGGL(k_deposit_cube<double>, dim3((unsigned)grid_items), dim3(256), 0, c.stream, pos_src,
-                               c.dep_start, item_start, nodes, c.dep_level, c.n, c.p.box_size, mass, c.grid);
+            hipLaunchKernelGGL(k_deposit_cube<double>, dim3((unsigned)grid_items), dim3(kDepThreads), 0, c.stream,
+                               pos_src, c.dep_start, item_start, nodes, c.dep_level, c.n, c.p.box_size, mass, c.grid);
         TPM_HIP(hipGetLastError());
         return;
     }

@@ -134,7 +134,7 @@ static void create(Ctx &c, const tpm_params &p, size_t max_n) {
     TPM_HIP(hipMalloc(&c.walk_ctr, 8 * 1024 * sizeof(uint32_t)));
     TPM_HIP(hipMalloc(&c.walk_goff, 64 * sizeof(uint32_t)));
     c.dep_level = 0;
-    while (((c.n + (1 << c.dep_level) - 1) >> c.dep_level) > 8) ++c.dep_level;
+    while (((c.n + (1 << c.dep_level) - 1) >> c.dep_level) > kDepSide) ++c.dep_level;
     {
         const size_t nodes1 = (size_t(1) << (3 * c.dep_level)) + 1;
         TPM_HIP(hipMalloc(&c.dep_start, 3 * nodes1 * sizeof(uint32_t)));

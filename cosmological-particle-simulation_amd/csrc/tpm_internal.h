@@ -33,6 +33,8 @@ constexpr int kMaxDepth = kKeyBits;
 // The group walk never opens a node of at most kWalkLeaf particles: it takes it as a monopole or
 // lists its particles. Children of such nodes are built only for the per-particle walks.
 constexpr uint32_t kWalkLeaf = 16;
+// The deposit's Morton cubes are the largest aligned cubes of at most kDepSide mesh cells a side.
+constexpr int kDepSide = 16;
 // Particles per walk group: one wavefront.
 constexpr int kGroup = 64;
 
@@ -186,7 +188,7 @@ struct Ctx {
     uint32_t *walk_ctr = nullptr;     // walk group counters, one per XCD range, 1024 words apart (8192 words)
     uint32_t *walk_goff = nullptr;    // first global group of each box of a batched walk (kWalkBatch + 1)
     uint32_t *dep_start = nullptr;    // per deposit cube: first sorted index, item count, first item (3 (8^dep_level + 1))
-    int dep_level = 0;                // Morton level of the deposit cubes (at most 8 cells a side)
+    int dep_level = 0;                // Morton level of the deposit cubes (at most kDepSide cells a side)
 };
 
 // ---------------------------------------------------------------------------------------
