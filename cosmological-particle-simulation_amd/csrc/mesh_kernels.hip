@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(256) k_deposit(const double *__restrict__ pos,
 // global atomic. A particle whose stencil lies outside the tile (only possible for a position
 // outside [0, L)) adds directly to the grid. Integer sums commute, so the grid is bit-identical to
 // k_deposit's for any order and any split. A cube of 16 cells a side holds about 4,096 particles
-// at the mean density: its tile of at most 21^3 cells (74 KB, two workgroups per CU) is cleared
+// at the mean density: its tile of at most 21^3 cells (74 KB) is cleared
 // and flushed once per 4,096 particles, about 1.7 tile cells per particle against 3.4 for cubes of
 // 8 cells, and the workgroup's fixed costs (the cube's bounds, the flush's reciprocals) are spread
 // over eight times as many particles.
