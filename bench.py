@@ -294,8 +294,9 @@ def ensemble_bench(args):
             "particles": total, "kdk_steps": steps, "theta": args.ensemble_theta,
             "force_evals": evals, "ms_graph": ms_graph, "ms_eager_tuned": ms_eager,
             "evals_per_s_eager_tuned": evals / (ms_eager / 1000.0),
-            "modes": "eager: each evaluation's walks batched into one launch per box side; graph: one launch of the "
-                     "captured run, a chain of single-box evaluations per box",
+            "modes": "eager: each evaluation's walks batched into one launch per box side, every box on its own "
+                     "stream; graph: one launch of the captured run, one stream per box side, the same batched "
+                     "walks",
             "ms_eager_default": ms_eager_default, "evals_per_s_graph": evals / (ms_graph / 1000.0),
             "graph_vs_eager_default": ms_eager_default / ms_graph, "setup_s": setup_s, "tune_s": tune_s,
             "first_graph_launch_s": capture_s, "graph_nodes": nodes, "tuning": choices,

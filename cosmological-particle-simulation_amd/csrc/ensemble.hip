@@ -3,8 +3,9 @@
 // No reference counterpart (the reference runs one box): an ensemble holds one context per box
 // (its own mesh, tree scratch and stream) plus device positions, momenta and accelerations.
 // A run (initial force and `steps` KDK steps for every box) is captured once into a HIP graph:
-// the boxes' work is forked onto their own streams from one capture stream and joined back, so
-// one graph launch executes the whole ensemble with the boxes overlapping on the GPU. Every
+// the work of each box side is forked onto its own stream from one capture stream and joined
+// back, so one graph launch executes the whole ensemble with the sides overlapping on the GPU and
+// the walks of one side's boxes batched into one launch per evaluation. Every
 // operation of a force evaluation is a kernel launch (no host synchronisation, no memset: the sort
 // and the scans are the library's own kernels), which is what makes it capturable.
 //
@@ -456,9 +457,9 @@ int tpm_ensemble_autotune(tpm_ensemble *e, double a, tpm_tuning *out) {
     });
 }
 
-// Enqueues initial force + `steps` KDK steps of every box: eagerly (use_graph = 0; the walks of the
-// boxes of one side batched into one launch per evaluation) or by one launch of the captured graph
-// (per-box chains).
+// Enqueues initial force + `steps` KDK steps of every box: eagerly (use_graph = 0) or by one launch
+// of the captured graph. Either way the walks of the boxes of one side run in one launch per
+// evaluation.
 int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start, double a_end, int steps,
                      int use_graph, double *ms) {
     return eguard(e, [&](tpm_ensemble &E) {
@@ -473,22 +474,6 @@ int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start
         std::vector<double> a(steps + 1);
         a[0] = a_start;
         for (int i = 0; i < steps; ++i) a[i + 1] = a_start * std::pow(a_end / a_start, (double)(i + 1) / steps);
-        // Per-box chains (the captured form): the whole run of each box on its own stream, each
-        // evaluation with its own walk. A captured run built from the batched form below (the walks
-        // of one side joined across the boxes' streams at every evaluation) crashed the host process
-        // inside the HIP runtime at 96 boxes and 21 evaluations (no GPU fault; small ensembles ran
-        // correctly), so graph mode keeps the per-box chains.
-        auto enqueue = [&](tpm::Box &bx) {
-            Ctx &c = tpm_ctx_internal(bx.ctx);
-            tpm::evaluate(c, bx.pos, bx.n, a[0], false, bx.acc, nullptr);
-            for (int i = 0; i < steps; ++i) {
-                const tpm::KdkFactors f = tpm::kdk_factors(*cosmo, a[i], a[i + 1]);
-                tpm::kick(c, bx.mom, bx.acc, bx.n, f.k1);
-                tpm::drift(c, bx.pos, bx.mom, bx.n, f.d);
-                tpm::evaluate(c, bx.pos, bx.n, a[i + 1], false, bx.acc, nullptr);
-                tpm::kick(c, bx.mom, bx.acc, bx.n, f.k2);
-            }
-        };
         // Boxes of one side share every scalar of the walk (split, softening and coupling in cells, the
         // tuned knobs): each evaluation walks them in one launch, on the stream of the side's first
         // box, once every box of the side has built its tree and groups. Everything else of an
@@ -553,6 +538,55 @@ int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start
                 TPM_HIP(hipStreamWaitEvent(E.master, done, 0));
             }
         };
+        // The captured form: one stream per side (its first box's stream). The side's boxes take
+        // their evaluations in turn on it, each evaluation's walks in one batched launch between,
+        // and the sides overlap one another; the streams meet only at the start and the end. (A
+        // capture of the form above, whose walks join the boxes' streams at every evaluation,
+        // crashed the host process inside the HIP runtime at 96 boxes and 21 evaluations, with no
+        // GPU fault, while small ensembles ran correctly in it.) The boxes' contexts use the side's
+        // stream while enqueuing and get their own back afterwards, also on an error.
+        auto enqueue_sides = [&](hipEvent_t start) {
+            struct Streams {
+                tpm_ensemble &e;
+                std::vector<hipStream_t> own;
+                explicit Streams(tpm_ensemble &ee) : e(ee) {
+                    for (auto &bx : e.boxes) own.push_back(tpm_ctx_internal(bx.ctx).stream);
+                }
+                ~Streams() {
+                    for (size_t b = 0; b < own.size(); ++b) tpm_ctx_internal(e.boxes[b].ctx).stream = own[b];
+                }
+            } restore(E);
+            for (const auto &sd : sides) {
+                Ctx &lead = tpm_ctx_internal(E.boxes[sd[0]].ctx);
+                const hipStream_t s = lead.stream;
+                TPM_HIP(hipStreamWaitEvent(s, start, 0));
+                for (size_t b : sd) tpm_ctx_internal(E.boxes[b].ctx).stream = s;
+                std::vector<tpm::WalkArgs> batch(sd.size());
+                for (int i = 0; i <= steps; ++i) {
+                    tpm::KdkFactors f{};
+                    if (i > 0) f = tpm::kdk_factors(*cosmo, a[i - 1], a[i]);
+                    for (size_t q = 0; q < sd.size(); ++q) {
+                        tpm::Box &bx = E.boxes[sd[q]];
+                        Ctx &c = tpm_ctx_internal(bx.ctx);
+                        if (i > 0) {
+                            tpm::kick(c, bx.mom, bx.acc, bx.n, f.k1);
+                            tpm::drift(c, bx.pos, bx.mom, bx.n, f.d);
+                        }
+                        batch[q] = tpm::evaluate_pre_walk(c, bx.pos, bx.n, a[i], bx.acc);
+                    }
+                    tpm::walk_launch_batch(lead, batch);
+                    for (size_t q = 0; q < sd.size(); ++q) {
+                        tpm::Box &bx = E.boxes[sd[q]];
+                        Ctx &c = tpm_ctx_internal(bx.ctx);
+                        tpm::evaluate_post_walk(c, bx.acc, bx.n);
+                        if (i > 0) tpm::kick(c, bx.mom, bx.acc, bx.n, f.k2);
+                    }
+                }
+                hipEvent_t done = new_event();
+                TPM_HIP(hipEventRecord(done, s));
+                TPM_HIP(hipStreamWaitEvent(E.master, done, 0));
+            }
+        };
         hipEvent_t t0, t1;
         TPM_HIP(hipEventCreate(&t0));
         TPM_HIP(hipEventCreate(&t1));
@@ -581,13 +615,7 @@ int tpm_ensemble_run(tpm_ensemble *e, const tpm_cosmology *cosmo, double a_start
                 hipEvent_t fork = new_event();
                 TPM_HIP(hipStreamBeginCapture(E.master, hipStreamCaptureModeRelaxed));
                 TPM_HIP(hipEventRecord(fork, E.master));
-                for (size_t b = 0; b < E.boxes.size(); ++b) {
-                    TPM_HIP(hipStreamWaitEvent(stream_of(b), fork, 0));
-                    enqueue(E.boxes[b]);
-                    hipEvent_t done = new_event();
-                    TPM_HIP(hipEventRecord(done, stream_of(b)));
-                    TPM_HIP(hipStreamWaitEvent(E.master, done, 0));
-                }
+                enqueue_sides(fork);
                 TPM_HIP(hipStreamEndCapture(E.master, &E.graph));
                 // Every operation of a run must be a kernel (memset nodes have given wrong results on
                 // replays after the first on these machines): count the node types and refuse a
