@@ -68,7 +68,7 @@ def test_deposit_bit_exact_and_order_independent(treepm):
 def test_dense_cube_deposit_in_evaluation(treepm, oracle):
     # The evaluation's deposit over Morton cubes takes the run-aggregated path for a cube of more
     # than one work item: 20,000 particles crowded into about one cell of a 32^3 mesh put far more
-    # than 2,048 particles into one 8-cell cube. Its density contrast must match the oracle's (FP32
+    # than 8,192 particles into one 16-cell cube. Its density contrast must match the oracle's (FP32
     # weights and FP64 weights), and the integer grid must not depend on the input order.
     rng = np.random.default_rng(5)
     box, n = 100.0, 32
@@ -91,6 +91,21 @@ def test_dense_cube_deposit_in_evaluation(treepm, oracle):
     ref64 = oracle.density_contrast(pos, n, box, dtype=np.float64)
     assert np.max(np.abs(d64 - ref64) / np.maximum(np.abs(ref64 + 1), 1.0)) < 1e-6
     assert d32.max() > 1000.0  # the clump really is dense
+
+
+@pytest.mark.parametrize("n", [5, 12, 17, 20, 33, 48, 96])
+def test_cube_deposit_exact_at_uneven_meshes(treepm, n):
+    # Meshes whose side is no multiple of the deposit cube's (17, 33: cubes of 8.5 and 8.25 cells,
+    # so the tiles take a ceil), meshes smaller than a tile (5, 12: one cube, tile cells wrapping
+    # onto the same grid cells), and meshes of several whole cubes (20, 48, 96). At n = 12 the
+    # single cube holds 20,000 particles: a dense cube of three work items. The evaluation's
+    # density contrast must equal the exact integer sum of the reference's FP32 quanta bit for bit.
+    rng = np.random.default_rng(100 + n)
+    box = 100.0
+    pos32 = (rng.random((20000, 3)) * box).astype(np.float32)
+    s = solver(treepm, n, box, max_particles=len(pos32))
+    _, d32 = s.accelerations(pos32, 0.5, want_delta=True)
+    assert np.array_equal(d32.view(np.uint32), fixed_point_delta32(pos32, n, box).view(np.uint32))
 
 
 def test_density_contrast_matches_oracle(treepm, oracle):
