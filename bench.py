@@ -307,7 +307,7 @@ def ensemble_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=512, help="particles per side (= mesh cells per side)")
     ap.add_argument("--workload", choices=["zeldovich", "nfw"], default="zeldovich",
@@ -419,6 +419,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        # Median of the timed evaluations' own device times (HIP events around each evaluation).
+        "ms_per_step_median_device": float(np.median([st["ms_total"] for st in stats])),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
