@@ -1,8 +1,7 @@
 """GPU parity suite: libtpm's HIP path against the CPU oracle, the golden vectors and the
 reference's own known-answer tests, at sizes the oracle finishes in seconds.
 
-Tolerances (north star: mass grid to 1e-6 relative, fixed-point grid bit-exact run to run;
-forces in FP32):
+Tolerances (mass grid to 1e-6 relative, fixed-point grid bit-exact run to run; forces in FP32):
   deposit   |rho - rho_ref| <= 1e-6 max(rho_ref, mean rho) per cell; bit-exact run to run and
             under any permutation of the particles
   PM force  within 2e-5 of the field's maximum vs the FP64 oracle (FP32 FFT round-off)
