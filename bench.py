@@ -331,6 +331,8 @@ def main():
                     default=[0.35, 0.5, 0.7, 1.0])
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default knobs)")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe-inclusive) evaluation")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run every stage on one stream (each kernel timed alone; the forces do not change)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-boxes", type=int, default=96)
     ap.add_argument("--ensemble-steps", type=int, default=20)
@@ -362,6 +364,8 @@ def main():
     log(f"[bench] {npart} particles, mesh {n}^3, theta {args.theta}")
 
     solver = treepm.Solver(n, BOX, OMEGA_M, split=split, max_particles=npart)
+    if args.no_overlap:
+        solver.set_overlap(False)
     pos = treepm.DeviceArray(3 * npart, np.float64)
     acc = treepm.DeviceArray(3 * npart, np.float32)
     if args.workload == "zeldovich":
@@ -432,6 +436,7 @@ def main():
                    "softening": split["softening"], "positions": "f64", "pair_and_mesh_arithmetic": "f32"},
         "tuning": tuning,
         "stages_ms": stages,
+        "octree_beside_pm": not args.no_overlap,
         "interactions_per_particle": inter / npart,
         "walk_sources_per_group": {"groups": mean("groups"), "listed": mean("listed") / max(mean("groups"), 1),
                                    "kept": mean("kept") / max(mean("groups"), 1)},

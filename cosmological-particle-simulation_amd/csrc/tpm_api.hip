@@ -285,7 +285,7 @@ void evaluate(Ctx &c, const double *d_pos, size_t n, double a, bool f32w, float 
     tm.mark();
     WalkArgs wa{};
     std::unique_ptr<SideStream> side;
-    if (walk && n > kOverlapMin) {
+    if (walk && n > kOverlapMin && c.overlap) {
         side.reset(new SideStream(c));
         wa = tree_chain(c, n, k, d_acc, group_walk);
         side->leave();
@@ -357,7 +357,7 @@ WalkArgs evaluate_pre_walk(Ctx &c, const double *d_pos, size_t n, double a, floa
     sort_particles(c, d_pos, n);
     std::unique_ptr<SideStream> side;
     WalkArgs w{};
-    if (n > kOverlapMin) {
+    if (n > kOverlapMin && c.overlap) {
         side.reset(new SideStream(c));
         w = tree_chain(c, n, coupling(c, a, n, false), d_acc, true);
         side->leave();
@@ -561,6 +561,13 @@ int tpm_set_timing(tpm_ctx *ctx, int enabled) {
 int tpm_set_counting(tpm_ctx *ctx, int enabled) {
     return guarded(ctx, [&](Ctx &c) {
         c.count_pairs = enabled != 0;
+        return TPM_OK;
+    });
+}
+
+int tpm_set_overlap(tpm_ctx *ctx, int enabled) {
+    return guarded(ctx, [&](Ctx &c) {
+        c.overlap = enabled != 0;
         return TPM_OK;
     });
 }

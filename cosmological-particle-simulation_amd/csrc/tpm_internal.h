@@ -121,6 +121,7 @@ struct Ctx {
     hipEvent_t tree_ev[2] = {nullptr, nullptr};  // around the tree chain when timing
     bool timing = false;
     bool count_pairs = true;  // walk interactions counted (tpm_set_counting)
+    bool overlap = true;      // octree beside the PM part on the second stream (tpm_set_overlap)
     std::string err;
     tpm_stats stats{};
 

@@ -43,7 +43,8 @@ STATUS_NONFINITE = 4
 EXPORTS = [
     "tpm_version", "tpm_create", "tpm_destroy", "tpm_last_error", "tpm_set_params",
     "tpm_accelerations_f32", "tpm_accelerations_f64", "tpm_accelerations_dev_f64",
-    "tpm_get_stats", "tpm_set_timing", "tpm_set_counting", "tpm_deposit_f32", "tpm_deposit_f64",
+    "tpm_get_stats", "tpm_set_timing", "tpm_set_counting", "tpm_set_overlap", "tpm_deposit_f32",
+    "tpm_deposit_f64",
     "tpm_density_contrast_f32", "tpm_pm_accel", "tpm_interpolate_f32",
     "tpm_short_range_f32", "tpm_short_range_f64", "tpm_judge_f64",
     "tpm_zeldovich", "tpm_zeldovich_dev", "tpm_kdk_factors", "tpm_step_to_dev", "tpm_run",
@@ -218,6 +219,7 @@ def load():
         "tpm_get_stats": (ctypes.c_int, [P, ctypes.POINTER(Stats)]),
         "tpm_set_timing": (ctypes.c_int, [P, ctypes.c_int]),
         "tpm_set_counting": (ctypes.c_int, [P, ctypes.c_int]),
+        "tpm_set_overlap": (ctypes.c_int, [P, ctypes.c_int]),
         "tpm_deposit_f32": (ctypes.c_int, [P, P, S, F, P]),
         "tpm_deposit_f64": (ctypes.c_int, [P, P, S, D, P]),
         "tpm_density_contrast_f32": (ctypes.c_int, [P, P, S, P]),
@@ -365,6 +367,10 @@ class Solver:
     def set_counting(self, on=True):
         """Interaction counting in the walk (on by default); off, stats()["interactions"] reads 0."""
         self._check(self.lib.tpm_set_counting(self.h, 1 if on else 0))
+
+    def set_overlap(self, on=True):
+        """The octree on a second stream beside the PM part above 2^22 particles (on by default)."""
+        self._check(self.lib.tpm_set_overlap(self.h, 1 if on else 0))
 
     def stats(self):
         s = Stats()

@@ -129,6 +129,10 @@ int tpm_set_timing(tpm_ctx *ctx, int enabled);
 /* Interaction counting in the group walk (no reference counterpart): on by default; off, the walk
  * saves one vector instruction per particle pair and tpm_stats.interactions reads 0. */
 int tpm_set_counting(tpm_ctx *ctx, int enabled);
+/* The octree on a second stream beside the deposit, FFT and interpolation, for evaluations of more
+ * than 2^22 particles (no reference counterpart): on by default; off, every stage runs on the
+ * context's stream in turn (same forces, bit for bit; each kernel's time measured alone). */
+int tpm_set_overlap(tpm_ctx *ctx, int enabled);
 
 /*
  * The particle order of the octree: Tree::build's counting sort into octants (src/tree.rs:215-236)

@@ -327,6 +327,33 @@ def test_force_evaluation_is_bit_exact_run_to_run(treepm):
     assert np.array_equal(d1.view(np.uint32), d2.view(np.uint32))
 
 
+def test_second_stream_does_not_change_the_forces(treepm):
+    # Above 2^22 particles the octree and walk groups run on a second stream beside the deposit,
+    # FFT and interpolation; with that switched off every stage runs on one stream in turn. The
+    # forces must be the same bit for bit, and so must a second evaluation with the overlap back on.
+    n = 192  # 7.1M particles
+    npart = n ** 3
+    split = treepm.force_split(100.0, n, n, 1 / 30, 0.35)
+    s = solver(treepm, n, 100.0, split, max_particles=npart)
+    d_pos = treepm.DeviceArray(3 * npart, np.float64)
+    d_acc = treepm.DeviceArray(3 * npart, np.float32)
+    s.zeldovich_dev(1, 2.0, 0.3 * 100.0 / n, 0.0, d_pos.ptr)
+    s.accelerations_dev(d_pos.ptr, npart, 0.02, d_acc.ptr)
+    s.sync()
+    beside = d_acc.numpy()
+    s.set_overlap(False)
+    s.accelerations_dev(d_pos.ptr, npart, 0.02, d_acc.ptr)
+    s.sync()
+    alone = d_acc.numpy()
+    s.set_overlap(True)
+    s.accelerations_dev(d_pos.ptr, npart, 0.02, d_acc.ptr)
+    s.sync()
+    again = d_acc.numpy()
+    assert np.all(np.isfinite(beside)) and np.abs(beside).max() > 0
+    assert np.array_equal(beside.view(np.uint32), alone.view(np.uint32))
+    assert np.array_equal(beside.view(np.uint32), again.view(np.uint32))
+
+
 def test_group_building_paths_agree(treepm):
     # Walk groups come from per-cube starts when the cube count fits the context's scratch and from
     # scans over all particles otherwise; both give the same groups, hence bit-identical forces.
